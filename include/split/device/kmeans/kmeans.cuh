@@ -1,0 +1,11 @@
+// Drop-in for split/include/split/device/kmeans/kmeans.cuh:4-8
+// (propagate_centroids stays on its existing path and is not included).
+#if !defined(SPLIT_DEVICE_INCLUDED_KMEANS)
+#define SPLIT_DEVICE_INCLUDED_KMEANS
+
+#include "split/device/kmeans/uniform_random_initialize.cuh"
+#include "split/device/kmeans/centroids.cuh"
+#include "split/device/kmeans/label.cuh"
+#include "split/device/kmeans/cluster.cuh"
+
+#endif  // SPLIT_DEVICE_INCLUDED_KMEANS

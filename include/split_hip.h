@@ -1,0 +1,180 @@
+/*
+ * split_hip.h — C ABI of the MI355X (gfx950) segmentation hot path.
+ *
+ * Plain pointers and sizes only: every buffer is caller-owned device memory
+ * (hipMalloc'd), points are three SoA float planes of n pixels each
+ * (channel c of pixel i at points[c*n + i]), centroids are a 3 x k row-major
+ * float matrix (channel c of centroid j at centroids[c*k + j]), labels are
+ * int32.  `stream` is a hipStream_t (NULL = the default stream).  Every entry
+ * point returns SPLIT_HIP_OK (0) or a nonzero status; the thread-local text of
+ * the last failure is available from split_hip_last_error().
+ *
+ * Each entry point names the reference interface it replaces
+ * (nitronoid/split, paths relative to the repository root).  The C++
+ * drop-in headers under include/split/ wrap these functions with the
+ * reference's namespaces, names and argument order.
+ */
+#ifndef SPLIT_HIP_H
+#define SPLIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* split_hip_stream_t; /* == hipStream_t */
+
+enum split_hip_status {
+  SPLIT_HIP_OK = 0,
+  SPLIT_HIP_ERR_ARGUMENT = 1,    /* null pointer, bad size, k or dim       */
+  SPLIT_HIP_ERR_TEMP = 2,        /* temp_bytes below the *_workspace size  */
+  SPLIT_HIP_ERR_HIP = 3,         /* a HIP runtime call or launch failed    */
+  SPLIT_HIP_ERR_RANGE = 4,       /* label value range exceeds what temp holds */
+  SPLIT_HIP_ERR_UNSUPPORTED = 5  /* a size beyond a documented limit       */
+};
+
+/* Largest k accepted by the assignment kernels (centroids held in registers,
+ * 16 per lane). */
+#define SPLIT_HIP_MAX_K 1024
+
+/* Result of a Lloyd run. */
+typedef struct split_hip_cluster_result {
+  int32_t iterations; /* Lloyd iterations performed                          */
+  float shift;        /* Frobenius norm of the last centroid update           */
+} split_hip_cluster_result;
+
+/* Text of the last failure on the calling thread ("" if none). */
+const char* split_hip_last_error(void);
+/* Text of a status code. */
+const char* split_hip_status_string(int status);
+/* Library version, e.g. "0.1.0". */
+const char* split_hip_version(void);
+
+/* ---- k-means ---------------------------------------------------------- */
+
+/* Replaces split::device::kmeans::label_points_workspace
+ * (split/include/split/device/kmeans/label.cuh:39-40,
+ *  split/src/split/device/kmeans/label.cu:68-72).  The fused kernel keeps
+ * distances in registers, so the temp it needs is 0 bytes. */
+size_t split_hip_label_points_workspace(int64_t n, int32_t k);
+
+/* Replaces split::device::kmeans::label_points
+ * (split/include/split/device/kmeans/label.cuh:27-31,
+ *  split/src/split/device/kmeans/label.cu:74-106).
+ * labels[i] = argmin_j ((p0-c0)^2 + (p1-c1)^2) + (p2-c2)^2 over the k
+ * centroids, ties to the lowest j.  1 <= k <= SPLIT_HIP_MAX_K. */
+int split_hip_label_points(const float* centroids, int32_t k,
+                           const float* points, int64_t n, int32_t* labels,
+                           void* temp, size_t temp_bytes,
+                           split_hip_stream_t stream);
+
+/* Replaces split::device::kmeans::calculate_centroids_workspace
+ * (split/include/split/device/kmeans/centroids.cuh:39-40,
+ *  split/src/split/device/kmeans/centroids.cu:16-20). */
+size_t split_hip_calculate_centroids_workspace(int64_t n, int32_t k);
+
+/* Replaces split::device::kmeans::calculate_centroids
+ * (split/include/split/device/kmeans/centroids.cuh:26-30,
+ *  split/src/split/device/kmeans/centroids.cu:22-101).
+ * centroids[c*k+j] = mean of channel c over pixels labelled j; a cluster with
+ * no pixels gets the zero centroid (the reference's stated intent,
+ * centroids.cu:65-67).  Labels outside [0, k) contribute nothing.
+ * Counts are exact integers; sums are float, reduced in a fixed order. */
+int split_hip_calculate_centroids(const int32_t* labels, const float* points,
+                                  int64_t n, float* centroids, int32_t k,
+                                  void* temp, size_t temp_bytes,
+                                  split_hip_stream_t stream);
+
+/* Replaces split::device::kmeans::cluster_workspace
+ * (split/include/split/device/kmeans/cluster.cuh:46-47,
+ *  split/src/split/device/kmeans/cluster.cu:12-25).  dim must be 3. */
+size_t split_hip_cluster_workspace(int64_t n, int32_t k, int32_t dim);
+
+/* Replaces split::device::kmeans::cluster
+ * (split/include/split/device/kmeans/cluster.cuh:31-37,
+ *  split/src/split/device/kmeans/cluster.cu:27-93).
+ * Lloyd iterations (assign, then centroid update) starting from the
+ * centroids passed in; the final centroids are written back to `centroids`.
+ * Iteration stops once the Frobenius norm of a centroid update is
+ * <= threshold, or after max_iter iterations (threshold < 0 runs exactly
+ * max_iter).  As in the reference, `labels` holds the assignment made in the
+ * last iteration, i.e. against the centroids before the last update.
+ * Host-synchronous on return (like the reference, which reads the shift back
+ * every iteration); `result` may be NULL. */
+int split_hip_cluster(const float* points, int64_t n, float* centroids,
+                      int32_t k, int32_t* labels, void* temp,
+                      size_t temp_bytes, int32_t max_iter, float threshold,
+                      split_hip_cluster_result* result,
+                      split_hip_stream_t stream);
+
+/* Replaces split::device::kmeans::uniform_random_initialize
+ * (split/include/split/device/kmeans/uniform_random_initialize.cuh:20-22,
+ *  split/src/split/device/kmeans/uniform_random_initialize.cu:10-55)
+ * with a reproducible form: k distinct pixel indices drawn from SplitMix64
+ * seeded by `seed` (index = high 64 bits of draw * n, redrawn on a repeat),
+ * and their colours copied as the centroids.  The reference seeds with
+ * time(NULL) and can read index n; this form cannot.  k <= n. */
+int split_hip_uniform_random_initialize(const float* points, int64_t n,
+                                        float* centroids, int32_t k,
+                                        uint64_t seed,
+                                        split_hip_stream_t stream);
+
+/* ---- connected components -------------------------------------------- */
+
+/* Temp needed by split_hip_connected_components (the reference ignores its
+ * temp argument and allocates 2N+3R ints internally,
+ * split/src/split/device/ccl/connected_components.cu:93-94,140-142). */
+size_t split_hip_connected_components_workspace(int32_t width, int32_t height);
+
+/* Replaces split::device::ccl::connected_components
+ * (split/include/split/device/ccl/connected_components.cuh:25-29,
+ *  split/src/split/device/ccl/connected_components.cu:80-181).
+ * labels: height x width row-major cluster labels.  out[p] = the largest
+ * linear pixel index of p's connected component of equal labels — the value
+ * the reference's max-propagation reaches at convergence — under 4- or
+ * 8-connectivity.  The union-find always runs to completion, so max_iter is
+ * accepted for signature parity and ignored. */
+int split_hip_connected_components(const int32_t* labels, int32_t width,
+                                   int32_t height, void* temp,
+                                   size_t temp_bytes, int32_t* out,
+                                   int32_t max_iter, int32_t connectivity,
+                                   split_hip_stream_t stream);
+
+/* Temp needed by split_hip_compress_labels for n labels whose values span
+ * at most `value_span` distinct integers (max - min + 1); pass n for labels
+ * produced by split_hip_connected_components. */
+size_t split_hip_compress_labels_workspace(int64_t n, int64_t value_span);
+
+/* Replaces split::device::ccl::compress_labels
+ * (split/include/split/device/ccl/compress_labels.cuh:21-23,
+ *  split/src/split/device/ccl/compress_labels.cu:10-35).
+ * In place: labels[i] <- rank of labels[i] among the sorted distinct values.
+ * *num_unique receives the number of distinct values (host-synchronous).
+ * Fails with SPLIT_HIP_ERR_RANGE if max - min + 1 exceeds what temp holds. */
+int split_hip_compress_labels(int32_t* labels, int64_t n, void* temp,
+                              size_t temp_bytes, int32_t* num_unique,
+                              split_hip_stream_t stream);
+
+/* ---- whole segmentation path ----------------------------------------- */
+
+/* cluster -> connected_components -> compress_labels in one stream-ordered
+ * call: the caller sequence of samples/src/main.cu:240-252.  Writes the
+ * cluster labels (`labels`, as split_hip_cluster) and the compressed segment
+ * labels (`segments`); *num_segments gets the segment count.  `compress` = 0
+ * leaves `segments` as component representatives.  temp must hold
+ * split_hip_segment_workspace(width, height, k) bytes. */
+size_t split_hip_segment_workspace(int32_t width, int32_t height, int32_t k);
+int split_hip_segment(const float* points, int32_t width, int32_t height,
+                      float* centroids, int32_t k, int32_t* labels,
+                      int32_t* segments, void* temp, size_t temp_bytes,
+                      int32_t max_iter, float threshold, int32_t connectivity,
+                      int32_t compress, split_hip_cluster_result* result,
+                      int32_t* num_segments, split_hip_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPLIT_HIP_H */

@@ -1,0 +1,295 @@
+"""split_amd — MI355X (gfx950) segmentation hot path of nitronoid/split.
+
+The compute path is the C-ABI library ``split_amd/_lib/libsplit_hip.so``
+(hand-written HIP kernels); this module only loads it with ctypes and passes
+device pointers from torch tensors (torch provides device memory and the
+stream, nothing else).  There is no CPU fallback: if the library is missing,
+every entry point raises ``SplitLibraryError``.
+
+Entry points mirror the reference's C++ API (split/include/split/device/
+kmeans/*.cuh, ccl/{connected_components,compress_labels}.cuh):
+
+    kmeans.label_points(centroids, points, labels)
+    kmeans.calculate_centroids(labels, points, centroids)
+    kmeans.cluster(points, centroids, labels, max_iter, threshold) -> (iters, shift)
+    kmeans.uniform_random_initialize(points, centroids, seed)
+    ccl.connected_components(labels_hw, out, connectivity=4)
+    ccl.compress_labels(labels) -> number of distinct labels
+    segment(points, width, height, centroids, ...) -> SegmentResult
+
+Layouts: points (3, N) float32 SoA planes, centroids (3, K) float32,
+labels int32 — all contiguous CUDA (HIP) tensors.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from pathlib import Path
+from types import SimpleNamespace
+
+LIB_DIR = Path(__file__).resolve().parent / "_lib"
+HIP_LIB = LIB_DIR / "libsplit_hip.so"
+GEN_LIB = LIB_DIR / "libsplit_gen.so"
+
+
+class SplitLibraryError(RuntimeError):
+    """The gfx950 library is missing or failed to load."""
+
+
+class SplitError(RuntimeError):
+    """A C-ABI call returned a nonzero status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+class ClusterResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("shift", ctypes.c_float)]
+
+
+_c_void_p = ctypes.c_void_p
+_c_size_t = ctypes.c_size_t
+_c_i32 = ctypes.c_int32
+_c_i64 = ctypes.c_int64
+_c_u64 = ctypes.c_uint64
+_c_float = ctypes.c_float
+
+# name -> (restype, argtypes); exactly the declarations of include/split_hip.h
+CABI = {
+    "split_hip_last_error": (ctypes.c_char_p, []),
+    "split_hip_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "split_hip_version": (ctypes.c_char_p, []),
+    "split_hip_label_points_workspace": (_c_size_t, [_c_i64, _c_i32]),
+    "split_hip_label_points": (ctypes.c_int, [_c_void_p, _c_i32, _c_void_p, _c_i64,
+                                              _c_void_p, _c_void_p, _c_size_t, _c_void_p]),
+    "split_hip_calculate_centroids_workspace": (_c_size_t, [_c_i64, _c_i32]),
+    "split_hip_calculate_centroids": (ctypes.c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p,
+                                                     _c_i32, _c_void_p, _c_size_t, _c_void_p]),
+    "split_hip_cluster_workspace": (_c_size_t, [_c_i64, _c_i32, _c_i32]),
+    "split_hip_cluster": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p, _c_i32, _c_void_p,
+                                         _c_void_p, _c_size_t, _c_i32, _c_float,
+                                         ctypes.POINTER(ClusterResult), _c_void_p]),
+    "split_hip_uniform_random_initialize": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p,
+                                                           _c_i32, _c_u64, _c_void_p]),
+    "split_hip_connected_components_workspace": (_c_size_t, [_c_i32, _c_i32]),
+    "split_hip_connected_components": (ctypes.c_int, [_c_void_p, _c_i32, _c_i32, _c_void_p,
+                                                      _c_size_t, _c_void_p, _c_i32, _c_i32,
+                                                      _c_void_p]),
+    "split_hip_compress_labels_workspace": (_c_size_t, [_c_i64, _c_i64]),
+    "split_hip_compress_labels": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p, _c_size_t,
+                                                 ctypes.POINTER(_c_i32), _c_void_p]),
+    "split_hip_segment_workspace": (_c_size_t, [_c_i32, _c_i32, _c_i32]),
+    "split_hip_segment": (ctypes.c_int, [_c_void_p, _c_i32, _c_i32, _c_void_p, _c_i32,
+                                         _c_void_p, _c_void_p, _c_void_p, _c_size_t, _c_i32,
+                                         _c_float, _c_i32, _c_i32,
+                                         ctypes.POINTER(ClusterResult),
+                                         ctypes.POINTER(_c_i32), _c_void_p]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libsplit_hip.so (once).  Raises SplitLibraryError if absent."""
+    global _lib
+    if _lib is None:
+        if not HIP_LIB.exists():
+            raise SplitLibraryError(
+                f"{HIP_LIB} is missing: build it with `python -m split_amd.build` "
+                "(there is no CPU fallback)")
+        try:
+            handle = ctypes.CDLL(str(HIP_LIB))
+        except OSError as e:  # pragma: no cover - depends on the machine
+            raise SplitLibraryError(f"cannot load {HIP_LIB}: {e}") from e
+        for name, (res, args) in CABI.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        L = lib()
+        raise SplitError(status, f"{what}: {L.split_hip_status_string(status).decode()}: "
+                                 f"{L.split_hip_last_error().decode()}")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream() -> int:
+    torch = _torch()
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _dev(t, dtype, what: str) -> int:
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{what} must be a CUDA (HIP) tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{what} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    return t.data_ptr()
+
+
+def _temp(nbytes: int, like):
+    torch = _torch()
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=like.device)
+
+
+def _points(points) -> tuple[int, int]:
+    torch = _torch()
+    if points.dim() != 2 or points.shape[0] != 3:
+        raise ValueError("points must be (3, N)")
+    return _dev(points, torch.float32, "points"), points.shape[1]
+
+
+def _cent(centroids) -> tuple[int, int]:
+    torch = _torch()
+    if centroids.dim() != 2 or centroids.shape[0] != 3:
+        raise ValueError("centroids must be (3, K)")
+    return _dev(centroids, torch.float32, "centroids"), centroids.shape[1]
+
+
+# --------------------------------------------------------------- k-means ---
+
+def label_points(centroids, points, labels, temp=None) -> None:
+    """split::device::kmeans::label_points (label.cu:74-106)."""
+    torch = _torch()
+    c, k = _cent(centroids)
+    p, n = _points(points)
+    lb = _dev(labels, torch.int32, "labels")
+    if labels.numel() != n:
+        raise ValueError("labels must hold N entries")
+    _check(lib().split_hip_label_points(c, k, p, n, lb, None, 0, _stream()), "label_points")
+
+
+def calculate_centroids(labels, points, centroids, temp=None) -> None:
+    """split::device::kmeans::calculate_centroids (centroids.cu:22-101)."""
+    torch = _torch()
+    p, n = _points(points)
+    c, k = _cent(centroids)
+    lb = _dev(labels, torch.int32, "labels")
+    if labels.numel() != n:
+        raise ValueError("labels must hold N entries")
+    L = lib()
+    if temp is None:
+        temp = _temp(L.split_hip_calculate_centroids_workspace(n, k), points)
+    _check(L.split_hip_calculate_centroids(lb, p, n, c, k, temp.data_ptr(), temp.numel(),
+                                           _stream()), "calculate_centroids")
+
+
+def cluster(points, centroids, labels, max_iter: int, threshold: float = 1.0,
+            temp=None) -> tuple[int, float]:
+    """split::device::kmeans::cluster (cluster.cu:27-93).  Returns
+    (iterations, last shift); host-synchronous like the reference."""
+    torch = _torch()
+    p, n = _points(points)
+    c, k = _cent(centroids)
+    lb = _dev(labels, torch.int32, "labels")
+    if labels.numel() != n:
+        raise ValueError("labels must hold N entries")
+    L = lib()
+    if temp is None:
+        temp = _temp(L.split_hip_cluster_workspace(n, k, 3), points)
+    r = ClusterResult()
+    _check(L.split_hip_cluster(p, n, c, k, lb, temp.data_ptr(), temp.numel(), int(max_iter),
+                               float(threshold), ctypes.byref(r), _stream()), "cluster")
+    return int(r.iterations), float(r.shift)
+
+
+def uniform_random_initialize(points, centroids, seed: int) -> None:
+    """Seeded form of split::device::kmeans::uniform_random_initialize."""
+    p, n = _points(points)
+    c, k = _cent(centroids)
+    _check(lib().split_hip_uniform_random_initialize(p, n, c, k, int(seed), _stream()),
+           "uniform_random_initialize")
+
+
+kmeans = SimpleNamespace(label_points=label_points, calculate_centroids=calculate_centroids,
+                         cluster=cluster, uniform_random_initialize=uniform_random_initialize)
+
+
+# ------------------------------------------------------------------- ccl ---
+
+def connected_components(labels_hw, out, connectivity: int = 4, max_iterations: int = 10,
+                         temp=None) -> None:
+    """split::device::ccl::connected_components (connected_components.cu:80-181):
+    out[p] = largest linear index of p's component."""
+    torch = _torch()
+    if labels_hw.dim() != 2:
+        raise ValueError("labels must be (H, W)")
+    h, w = labels_hw.shape
+    li = _dev(labels_hw, torch.int32, "labels")
+    oi = _dev(out, torch.int32, "out")
+    if out.numel() != h * w:
+        raise ValueError("out must hold H*W entries")
+    _check(lib().split_hip_connected_components(li, w, h, None, 0, oi, int(max_iterations),
+                                                int(connectivity), _stream()),
+           "connected_components")
+
+
+def compress_labels(labels, temp=None) -> int:
+    """split::device::ccl::compress_labels (compress_labels.cu:10-35).  In
+    place; returns the number of distinct values (host-synchronous)."""
+    torch = _torch()
+    li = _dev(labels, torch.int32, "labels")
+    n = labels.numel()
+    L = lib()
+    if temp is None:
+        temp = _temp(max(L.split_hip_compress_labels_workspace(n, n), 4 * n), labels)
+    cnt = _c_i32(0)
+    _check(L.split_hip_compress_labels(li, n, temp.data_ptr(), temp.numel(),
+                                       ctypes.byref(cnt), _stream()), "compress_labels")
+    return int(cnt.value)
+
+
+ccl = SimpleNamespace(connected_components=connected_components,
+                      compress_labels=compress_labels)
+
+
+# --------------------------------------------------------------- segment ---
+
+@dataclass
+class SegmentResult:
+    iterations: int
+    shift: float
+    num_segments: int
+
+
+def segment_workspace(width: int, height: int, k: int) -> int:
+    return int(lib().split_hip_segment_workspace(width, height, k))
+
+
+def segment(points, width: int, height: int, centroids, labels, segments, *,
+            max_iter: int = 100, threshold: float = 1e-4, connectivity: int = 8,
+            compress: bool = True, temp=None) -> SegmentResult:
+    """cluster -> connected_components -> compress_labels, stream-ordered
+    (the caller sequence of samples/src/main.cu:240-252)."""
+    torch = _torch()
+    p, n = _points(points)
+    if n != width * height:
+        raise ValueError("points must hold width*height pixels")
+    c, k = _cent(centroids)
+    lb = _dev(labels, torch.int32, "labels")
+    sg = _dev(segments, torch.int32, "segments")
+    L = lib()
+    if temp is None:
+        temp = _temp(L.split_hip_segment_workspace(width, height, k), points)
+    r = ClusterResult()
+    ns = _c_i32(0)
+    _check(L.split_hip_segment(p, width, height, c, k, lb, sg, temp.data_ptr(), temp.numel(),
+                               int(max_iter), float(threshold), int(connectivity),
+                               int(bool(compress)), ctypes.byref(r), ctypes.byref(ns),
+                               _stream()), "segment")
+    return SegmentResult(int(r.iterations), float(r.shift), int(ns.value))
+
+
+def version() -> str:
+    return lib().split_hip_version().decode()

@@ -1,0 +1,136 @@
+// Shared helpers of the gfx950 kernels and their C-ABI launchers: status
+// handling, wave64 cross-lane reductions built on DPP row operations, and
+// the device-side bookkeeping structs that live in the caller's temp block.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "split_hip.h"
+
+namespace splithip {
+
+// ---------------------------------------------------------------- status ---
+
+void set_error(const char* fmt, ...);
+int fail(int status, const char* fmt, ...);
+
+#define SPLIT_HIP_CHECK(expr)                                                \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return ::splithip::fail(SPLIT_HIP_ERR_HIP, "%s failed: %s (%s:%d)",    \
+                              #expr, hipGetErrorString(e_), __FILE__,        \
+                              __LINE__);                                     \
+  } while (0)
+
+#define SPLIT_HIP_CHECK_LAUNCH(name)                                         \
+  do {                                                                       \
+    hipError_t e_ = hipGetLastError();                                       \
+    if (e_ != hipSuccess)                                                    \
+      return ::splithip::fail(SPLIT_HIP_ERR_HIP, "launch of %s failed: %s",  \
+                              name, hipGetErrorString(e_));                  \
+  } while (0)
+
+#define SPLIT_HIP_TRY(expr)                                                  \
+  do {                                                                       \
+    int s_ = (expr);                                                         \
+    if (s_ != SPLIT_HIP_OK) return s_;                                       \
+  } while (0)
+
+// Number of compute units of the current device (cached per device).
+int device_cu_count();
+
+constexpr int kWave = 64;
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ------------------------------------------------------ wave reductions ---
+//
+// Reductions run on all 64 lanes (callers keep EXEC full).  Inside each row
+// of 16 lanes: quad xor-1, quad xor-2, half-row mirror, row mirror — after
+// which every lane of a row holds the row result — then the four row results
+// are read with v_readlane and combined in a fixed order, so the result is
+// wave-uniform and bit-identical run to run.
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
+}
+
+__device__ __forceinline__ float readlane_f(float x, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+__device__ __forceinline__ int readlane_i(int x, int lane) {
+  return __builtin_amdgcn_readlane(x, lane);
+}
+__device__ __forceinline__ int readfirstlane_i(int x) {
+  return __builtin_amdgcn_readfirstlane(x);
+}
+
+constexpr int kDppQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int kDppRowMirror = 0x140;
+constexpr int kDppRowHalfMirror = 0x141;
+
+struct OpAdd {
+  __device__ float operator()(float a, float b) const { return a + b; }
+};
+struct OpMin {
+  __device__ float operator()(float a, float b) const { return fminf(a, b); }
+};
+struct OpMax {
+  __device__ float operator()(float a, float b) const { return fmaxf(a, b); }
+};
+
+template <class Op>
+__device__ __forceinline__ float wave_allreduce(float x, Op op) {
+  x = op(x, dpp_f<kDppQuadXor1>(x));
+  x = op(x, dpp_f<kDppQuadXor2>(x));
+  x = op(x, dpp_f<kDppRowHalfMirror>(x));
+  x = op(x, dpp_f<kDppRowMirror>(x));
+  return op(op(readlane_f(x, 0), readlane_f(x, 16)),
+            op(readlane_f(x, 32), readlane_f(x, 48)));
+}
+
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned x) {
+  x += (unsigned)dpp_i<kDppQuadXor1>((int)x);
+  x += (unsigned)dpp_i<kDppQuadXor2>((int)x);
+  x += (unsigned)dpp_i<kDppRowHalfMirror>((int)x);
+  x += (unsigned)dpp_i<kDppRowMirror>((int)x);
+  return ((unsigned)readlane_i((int)x, 0) + (unsigned)readlane_i((int)x, 16)) +
+         ((unsigned)readlane_i((int)x, 32) + (unsigned)readlane_i((int)x, 48));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// -------------------------------------------------- temp-block structs ---
+
+// Lloyd loop state, kept in the caller's temp block.  Written by the last
+// block of each step launch; read by every block of the next launch.
+struct LloydState {
+  unsigned arrive;  // blocks of the current launch that have finished
+  int cur;          // which of the two centroid buffers is current
+  int iter;         // Lloyd iterations completed
+  int done;         // set once converged or max_iter reached
+  int max_iter;
+  float threshold;
+  float shift;      // Frobenius norm of the last centroid update
+  int reserved;
+};
+
+// Host-mapped mailbox the last block of a step launch writes, so the host
+// can stop enqueuing iterations without a device->host copy.
+struct Mailbox {
+  int iter;
+  int done;
+  float shift;
+  int reserved;
+};
+
+}  // namespace splithip

@@ -1,0 +1,348 @@
+"""GPU parity of the gfx950 path against the CPU oracle, through the C ABI.
+
+Bars (from the segmentation path's parity rules):
+  * labels: identical to the float restatement of the reference's dense
+    argmin (same arithmetic), and identical to the FP64 argmin except where the
+    two nearest squared distances differ by < 1e-5 relative;
+  * centroids: within 1e-5 (absolute, colours in [0, 1]) of the FP64 mean;
+  * connected components: bit-exact (both sides emit the largest linear index
+    of each component; the canonical min-index form is compared as well);
+  * compress_labels: bit-exact dense rank.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import split_amd as sa
+from split_amd import workloads as W
+from tests.conftest import say
+
+pytestmark = pytest.mark.gpu
+
+CENT_TOL = 1e-5
+TIE_REL = 1e-5
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def plain(torch_cuda):
+    w = W.PLAIN_512
+    img = W.make_image(w.image)
+    return w, img
+
+
+def seeded_centroids(img, k, seed):
+    idx = oracle.sample_indices(seed, img.shape[1], k)
+    return np.ascontiguousarray(img[:, idx])
+
+
+# ----------------------------------------------------------- assignment ---
+
+@pytest.mark.parametrize("k", [1, 3, 8, 32, 64, 65, 100, 256, 300, 1024])
+def test_label_points_matches_reference_argmin(torch_cuda, plain, k):
+    torch = torch_cuda
+    w, img = plain
+    say(f"label_points plain_512 k={k}")
+    cent = seeded_centroids(img, k, 1000 + k)
+    P, C = dev(torch, img), dev(torch, cent)
+    L = torch.empty(img.shape[1], dtype=torch.int32, device="cuda")
+    sa.label_points(C, P, L)
+    got = host(L)
+    # same float arithmetic as the reference's dense argmin: exact
+    want = oracle.ref_label_points(cent, img)
+    assert np.array_equal(got, want)
+    # FP64 argmin up to near ties
+    l64, gap = oracle.label_f64(img, cent)
+    bad = (got != l64) & (gap >= TIE_REL)
+    assert not bad.any(), f"{bad.sum()} labels differ beyond the near-tie rule"
+
+
+@pytest.mark.parametrize("n", [1, 5, 255, 256, 257, 1000, 4097])
+def test_label_points_ragged_sizes(torch_cuda, n):
+    torch = torch_cuda
+    say(f"label_points ragged n={n}")
+    rng = np.random.default_rng(n)
+    img = rng.random((3, n), dtype=np.float32)
+    cent = rng.random((3, 7), dtype=np.float32)
+    L = torch.empty(n, dtype=torch.int32, device="cuda")
+    sa.label_points(dev(torch, cent), dev(torch, img), L)
+    assert np.array_equal(host(L), oracle.ref_label_points(cent, img))
+
+
+def test_label_points_unaligned_planes(torch_cuda):
+    # n % 4 != 0 puts planes 1 and 2 off 16-byte alignment: scalar path
+    torch = torch_cuda
+    say("label_points unaligned n=262147")
+    rng = np.random.default_rng(7)
+    n = 262147
+    img = rng.random((3, n), dtype=np.float32)
+    cent = rng.random((3, 16), dtype=np.float32)
+    L = torch.empty(n, dtype=torch.int32, device="cuda")
+    sa.label_points(dev(torch, cent), dev(torch, img), L)
+    assert np.array_equal(host(L), oracle.ref_label_points(cent, img))
+
+
+def test_label_points_ties_and_duplicates(torch_cuda):
+    torch = torch_cuda
+    say("label_points ties")
+    # duplicate centroids: the lower index takes every pixel
+    img = np.random.default_rng(3).random((3, 4096), dtype=np.float32)
+    c = np.random.default_rng(4).random((3, 5), dtype=np.float32)
+    c[:, 3] = c[:, 1]
+    L = torch.empty(4096, dtype=torch.int32, device="cuda")
+    sa.label_points(dev(torch, c), dev(torch, img), L)
+    got = host(L)
+    assert not (got == 3).any()
+    assert np.array_equal(got, oracle.ref_label_points(c, img))
+    # pixels equal to a centroid get that centroid
+    c2 = np.ascontiguousarray(img[:, [10, 20, 30]])
+    sa.label_points(dev(torch, c2), dev(torch, img), L)
+    got = host(L)
+    assert got[10] == 0 and got[20] == 1 and got[30] == 2
+
+
+# -------------------------------------------------------------- centroids ---
+
+@pytest.mark.parametrize("k", [1, 8, 32, 256, 1024])
+def test_calculate_centroids_fp64(torch_cuda, plain, k):
+    torch = torch_cuda
+    w, img = plain
+    say(f"calculate_centroids plain_512 k={k}")
+    cent = seeded_centroids(img, min(k, 1024), 2000 + k)
+    labels = oracle.ref_label_points(cent, img)
+    out = torch.empty((3, k), dtype=torch.float32, device="cuda")
+    sa.calculate_centroids(dev(torch, labels), dev(torch, img), out)
+    got = host(out)
+    want, counts = oracle.centroids_f64(labels, img, k)
+    assert np.abs(got - want).max() <= CENT_TOL
+    assert np.all(got[:, counts == 0] == 0)
+
+
+def test_calculate_centroids_empty_and_out_of_range(torch_cuda):
+    torch = torch_cuda
+    say("calculate_centroids empty/out-of-range labels")
+    rng = np.random.default_rng(11)
+    n, k = 10000, 6
+    img = rng.random((3, n), dtype=np.float32)
+    labels = rng.integers(-2, 9, n).astype(np.int32)
+    labels[labels == 2] = 3  # label 2 empty
+    out = torch.empty((3, k), dtype=torch.float32, device="cuda")
+    sa.calculate_centroids(dev(torch, labels), dev(torch, img), out)
+    got = host(out)
+    want, counts = oracle.centroids_f64(labels, img, k)
+    assert counts[2] == 0
+    assert np.all(got[:, 2] == 0)
+    assert np.abs(got - want).max() <= CENT_TOL
+
+
+def test_calculate_centroids_closed_forms(torch_cuda):
+    torch = torch_cuda
+    say("calculate_centroids closed forms")
+    n, k = 3 * 4096 + 5, 3
+    # constant image -> constant centroid
+    img = np.full((3, n), 0.25, dtype=np.float32)
+    labels = (np.arange(n) % k).astype(np.int32)
+    out = torch.empty((3, k), dtype=torch.float32, device="cuda")
+    sa.calculate_centroids(dev(torch, labels), dev(torch, img), out)
+    assert np.all(host(out) == 0.25)
+    # affine ramp x_i = i/n with labels i mod k: mean of {j, j+k, ...}/n
+    ramp = (np.arange(n, dtype=np.float64) / n).astype(np.float32)
+    img = np.stack([ramp, 1 - ramp, ramp * 0.5]).astype(np.float32)
+    sa.calculate_centroids(dev(torch, labels), dev(torch, img), out)
+    got = host(out)
+    for j in range(k):
+        idx = np.arange(j, n, k)
+        assert abs(got[0, j] - img[0, idx].astype(np.float64).mean()) <= CENT_TOL
+
+
+# ------------------------------------------------------------------ Lloyd ---
+
+def test_cluster_fixed_iterations_plain_512(torch_cuda, plain):
+    torch = torch_cuda
+    w, img = plain
+    say("cluster plain_512 k=8 x20")
+    init = seeded_centroids(img, w.k, w.init_seed)
+    C = dev(torch, init)
+    L = torch.empty(img.shape[1], dtype=torch.int32, device="cuda")
+    it, shift = sa.cluster(dev(torch, img), C, L, w.max_iter, w.threshold)
+    assert it == 20
+    got_c, got_l = host(C), host(L)
+    want_c, want_l = oracle.lloyd_f64(img, init, 20)
+    assert np.abs(got_c - want_c).max() <= CENT_TOL
+    # labels of the last assignment: FP64 labels against the FP64 centroids
+    # after 19 updates
+    c19, _ = oracle.lloyd_f64(img, init, 19)
+    l64, gap = oracle.label_f64(img, c19.astype(np.float32))
+    bad = (got_l != want_l) & (gap >= TIE_REL)
+    assert not bad.any()
+    # teacher-forced: one GPU step from the GPU's own centroids equals the
+    # reference arithmetic exactly for labels and FP64 within 1e-5
+    L2 = torch.empty_like(L)
+    sa.label_points(C, dev(torch, img), L2)
+    lab = host(L2)
+    assert np.array_equal(lab, oracle.ref_label_points(got_c, img))
+    C2 = torch.empty_like(C)
+    sa.calculate_centroids(L2, dev(torch, img), C2)
+    assert np.abs(host(C2) - oracle.centroids_f64(lab, img, w.k)[0]).max() <= CENT_TOL
+
+
+def test_cluster_stops_on_threshold(torch_cuda, plain):
+    torch = torch_cuda
+    w, img = plain
+    say("cluster plain_512 to tolerance")
+    init = seeded_centroids(img, 8, 99)
+    C = dev(torch, init)
+    L = torch.empty(img.shape[1], dtype=torch.int32, device="cuda")
+    it, shift = sa.cluster(dev(torch, img), C, L, 100, 1e-4)
+    assert 1 <= it <= 100
+    assert shift <= 1e-4 or it == 100
+    # the FP64 trajectory reaches the same centroids after the same count
+    want_c, _ = oracle.lloyd_f64(img, init, it)
+    assert np.abs(host(C) - want_c).max() <= CENT_TOL
+
+
+def test_cluster_zero_iterations_leaves_inputs(torch_cuda, plain):
+    torch = torch_cuda
+    w, img = plain
+    say("cluster max_iter=0")
+    init = seeded_centroids(img, 8, 5)
+    C = dev(torch, init)
+    L = torch.full((img.shape[1],), -7, dtype=torch.int32, device="cuda")
+    it, _ = sa.cluster(dev(torch, img), C, L, 0, -1.0)
+    assert it == 0
+    assert np.array_equal(host(C), init)
+    assert np.all(host(L) == -7)
+
+
+def test_uniform_random_initialize_matches_sampler(torch_cuda, plain):
+    torch = torch_cuda
+    w, img = plain
+    say("uniform_random_initialize")
+    for k, seed in [(8, 1), (32, 2), (256, 3), (1024, 4)]:
+        C = torch.empty((3, k), dtype=torch.float32, device="cuda")
+        sa.uniform_random_initialize(dev(torch, img), C, seed)
+        idx = oracle.sample_indices(seed, img.shape[1], k)
+        assert len(set(idx.tolist())) == k
+        assert np.array_equal(host(C), img[:, idx])
+
+
+# ---------------------------------------------------- connected components ---
+
+def run_cc(torch, lab_hw, conn):
+    h, w = lab_hw.shape
+    out = torch.empty(h * w, dtype=torch.int32, device="cuda")
+    sa.connected_components(dev(torch, lab_hw), out, connectivity=conn)
+    return host(out)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("shape", [(1, 1), (1, 77), (77, 1), (31, 33), (64, 64),
+                                   (100, 37), (257, 129)])
+def test_cc_random_labels(torch_cuda, conn, shape):
+    torch = torch_cuda
+    say(f"cc random shape={shape} conn={conn}")
+    rng = np.random.default_rng(hash(shape) % 1000 + conn)
+    for nlab in (2, 3, 5):
+        lab = rng.integers(0, nlab, shape).astype(np.int32)
+        got = run_cc(torch, lab, conn)
+        want, _ = oracle.cc_uf(lab, conn)
+        assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_cc_voronoi_labels_and_reference_fixed_point(torch_cuda, plain, conn):
+    torch = torch_cuda
+    w, img = plain
+    say(f"cc plain_512 cluster labels conn={conn}")
+    cent = seeded_centroids(img, 8, 7)
+    lab = oracle.ref_label_points(cent, img).reshape(512, 512)
+    got = run_cc(torch, lab, conn)
+    want, _ = oracle.cc_uf(lab, conn)
+    assert np.array_equal(got, want)
+    assert np.array_equal(oracle.canonical_min(got), oracle.canonical_min(want))
+    if conn == 4:
+        ref, it = oracle.ref_connected_components(lab, 100000)
+        assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_cc_closed_forms(torch_cuda, conn):
+    torch = torch_cuda
+    say(f"cc closed forms conn={conn}")
+    h, w = 96, 130
+    uni = np.zeros((h, w), np.int32)
+    assert np.all(run_cc(torch, uni, conn) == h * w - 1)
+    cb = ((np.arange(h)[:, None] + np.arange(w)[None, :]) & 1).astype(np.int32)
+    got = run_cc(torch, cb, conn)
+    n_comp = len(np.unique(got))
+    assert n_comp == (h * w if conn == 4 else 2)
+    stripes = np.repeat((np.arange(h) % 2)[:, None], w, axis=1).astype(np.int32)
+    assert len(np.unique(run_cc(torch, stripes, conn))) == (h if conn == 4 else h)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_cc_spiral_stress_2048(torch_cuda, conn):
+    torch = torch_cuda
+    say(f"cc spiral/checkerboard 2048 conn={conn}")
+    lab = W.make_cc_stress(2048, 2048)
+    got = run_cc(torch, lab, conn)
+    want, ncomp = oracle.cc_uf(lab, conn)
+    assert np.array_equal(got, want)
+
+
+# ------------------------------------------------------------- compress ---
+
+def test_compress_labels_matches_sort_unique(torch_cuda):
+    torch = torch_cuda
+    say("compress_labels random")
+    rng = np.random.default_rng(5)
+    for lo, hi, n in [(0, 10, 1000), (-50, 50, 7777), (-1000000, 1000000, 100000),
+                      (5, 6, 1), (2**31 - 100, 2**31 - 1, 3000)]:
+        v = rng.integers(lo, hi, n, dtype=np.int64).astype(np.int32)
+        t = dev(torch, v)
+        cnt = sa.compress_labels(t)
+        want, wcnt = oracle.compress(v)
+        assert cnt == wcnt
+        assert np.array_equal(host(t), want)
+        assert np.array_equal(want, np.unique(v, return_inverse=True)[1].ravel())
+
+
+def test_compress_labels_range_error(torch_cuda):
+    torch = torch_cuda
+    say("compress_labels range error")
+    v = np.array([-2**31, 2**31 - 1, 0, 5], dtype=np.int32)
+    t = dev(torch, v)
+    tmp = torch.empty(64 + 4 * 4 + 64, dtype=torch.uint8, device="cuda")
+    with pytest.raises(sa.SplitError) as e:
+        sa.compress_labels(t, temp=tmp)
+    assert e.value.status == 4
+    assert np.array_equal(host(t), v)
+
+
+# ------------------------------------------------------------- pipeline ---
+
+def test_segment_plain_512(torch_cuda, plain):
+    torch = torch_cuda
+    w, img = plain
+    say("segment plain_512")
+    init = seeded_centroids(img, w.k, w.init_seed)
+    C = dev(torch, init)
+    L = torch.empty(w.n, dtype=torch.int32, device="cuda")
+    S = torch.empty(w.n, dtype=torch.int32, device="cuda")
+    r = sa.segment(dev(torch, img), 512, 512, C, L, S, max_iter=20, threshold=-1.0,
+                   connectivity=4)
+    lab = host(L).reshape(512, 512)
+    rep, ncomp = oracle.cc_uf(lab, 4)
+    want, cnt = oracle.compress(rep)
+    assert r.iterations == 20
+    assert r.num_segments == cnt == ncomp
+    assert np.array_equal(host(S), want)
