@@ -402,12 +402,22 @@ int compress_labels_async(int32_t* labels, int64_t n, void* temp,
   if (temp_bytes < min_need)
     return fail(SPLIT_HIP_ERR_TEMP, "temp holds %zu bytes, needs >= %zu",
                 temp_bytes, min_need);
-  // Largest bitmap the temp block holds: head + csum + 2 words per word.
-  int64_t cap = (int64_t)((temp_bytes - kCompressHead) / 8);
-  while (cap > 0 && kCompressHead + compress_csum_bytes(cap) +
-                            align_up((size_t)cap * 4, 16) * 2 >
-                        temp_bytes)
-    cap -= std::max<int64_t>(1, cap / 64);
+  // Largest bitmap (in 32-bit words) the temp block holds next to its
+  // prefix array and chunk sums.
+  auto fits = [&](int64_t w) {
+    return kCompressHead + compress_csum_bytes(w) +
+               2 * align_up((size_t)w * 4, 16) <=
+           temp_bytes;
+  };
+  int64_t lo = 1, hi = (int64_t)(temp_bytes / 8) + 1;  // fits(lo), !fits(hi)
+  while (hi - lo > 1) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (fits(mid))
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int64_t cap = lo;
   char* t = static_cast<char*>(temp);
   CompressState* st = reinterpret_cast<CompressState*>(t);
   unsigned* csum = reinterpret_cast<unsigned*>(t + kCompressHead);
