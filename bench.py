@@ -162,7 +162,7 @@ def main():
             torch.cuda.synchronize()
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record(stream)
-            sa.cluster(P, Ct, Lt, iters_, -1.0, temp=ctemp)
+            sa.cluster(P, Ct, Lt, iters_, -1.0, temp=ctemp, width=width, height=height)
             b.record(stream)
             torch.cuda.synchronize()
             return a.elapsed_time(b)

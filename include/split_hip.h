@@ -70,6 +70,16 @@ int split_hip_label_points(const float* centroids, int32_t k,
                            void* temp, size_t temp_bytes,
                            split_hip_stream_t stream);
 
+/* Image form of split_hip_label_points (same result): the caller also gives
+ * the row-major width x height layout of the n = width*height pixels, so the
+ * kernel can walk 16x16 tiles, whose colours are more compact than runs of a
+ * row and let more centroids be ruled out per tile.  The reference has no
+ * such overload; its callers hold the labels as a height x width array2d
+ * (samples/src/main.cu:234-237). */
+int split_hip_label_image(const float* centroids, int32_t k,
+                          const float* points, int32_t width, int32_t height,
+                          int32_t* labels, split_hip_stream_t stream);
+
 /* Replaces split::device::kmeans::calculate_centroids_workspace
  * (split/include/split/device/kmeans/centroids.cuh:39-40,
  *  split/src/split/device/kmeans/centroids.cu:16-20). */
@@ -108,6 +118,15 @@ int split_hip_cluster(const float* points, int64_t n, float* centroids,
                       size_t temp_bytes, int32_t max_iter, float threshold,
                       split_hip_cluster_result* result,
                       split_hip_stream_t stream);
+
+/* Image form of split_hip_cluster (same results) walking 16x16 tiles of the
+ * width x height image, as split_hip_label_image. */
+int split_hip_cluster_image(const float* points, int32_t width,
+                            int32_t height, float* centroids, int32_t k,
+                            int32_t* labels, void* temp, size_t temp_bytes,
+                            int32_t max_iter, float threshold,
+                            split_hip_cluster_result* result,
+                            split_hip_stream_t stream);
 
 /* Replaces split::device::kmeans::uniform_random_initialize
  * (split/include/split/device/kmeans/uniform_random_initialize.cuh:20-22,

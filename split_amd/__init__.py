@@ -63,6 +63,8 @@ CABI = {
     "split_hip_label_points_workspace": (_c_size_t, [_c_i64, _c_i32]),
     "split_hip_label_points": (ctypes.c_int, [_c_void_p, _c_i32, _c_void_p, _c_i64,
                                               _c_void_p, _c_void_p, _c_size_t, _c_void_p]),
+    "split_hip_label_image": (ctypes.c_int, [_c_void_p, _c_i32, _c_void_p, _c_i32, _c_i32,
+                                             _c_void_p, _c_void_p]),
     "split_hip_calculate_centroids_workspace": (_c_size_t, [_c_i64, _c_i32]),
     "split_hip_calculate_centroids": (ctypes.c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p,
                                                      _c_i32, _c_void_p, _c_size_t, _c_void_p]),
@@ -70,6 +72,10 @@ CABI = {
     "split_hip_cluster": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p, _c_i32, _c_void_p,
                                          _c_void_p, _c_size_t, _c_i32, _c_float,
                                          ctypes.POINTER(ClusterResult), _c_void_p]),
+    "split_hip_cluster_image": (ctypes.c_int, [_c_void_p, _c_i32, _c_i32, _c_void_p, _c_i32,
+                                               _c_void_p, _c_void_p, _c_size_t, _c_i32,
+                                               _c_float, ctypes.POINTER(ClusterResult),
+                                               _c_void_p]),
     "split_hip_uniform_random_initialize": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p,
                                                            _c_i32, _c_u64, _c_void_p]),
     "split_hip_connected_components_workspace": (_c_size_t, [_c_i32, _c_i32]),
@@ -159,15 +165,24 @@ def _cent(centroids) -> tuple[int, int]:
 
 # --------------------------------------------------------------- k-means ---
 
-def label_points(centroids, points, labels, temp=None) -> None:
-    """split::device::kmeans::label_points (label.cu:74-106)."""
+def label_points(centroids, points, labels, temp=None, *, width: int = 0,
+                 height: int = 0) -> None:
+    """split::device::kmeans::label_points (label.cu:74-106).  With width and
+    height (width*height == N) the kernel walks 16x16 tiles of the image."""
     torch = _torch()
     c, k = _cent(centroids)
     p, n = _points(points)
     lb = _dev(labels, torch.int32, "labels")
     if labels.numel() != n:
         raise ValueError("labels must hold N entries")
-    _check(lib().split_hip_label_points(c, k, p, n, lb, None, 0, _stream()), "label_points")
+    if width:
+        if width * height != n:
+            raise ValueError("width*height must equal N")
+        _check(lib().split_hip_label_image(c, k, p, width, height, lb, _stream()),
+               "label_image")
+    else:
+        _check(lib().split_hip_label_points(c, k, p, n, lb, None, 0, _stream()),
+               "label_points")
 
 
 def calculate_centroids(labels, points, centroids, temp=None) -> None:
@@ -186,9 +201,10 @@ def calculate_centroids(labels, points, centroids, temp=None) -> None:
 
 
 def cluster(points, centroids, labels, max_iter: int, threshold: float = 1.0,
-            temp=None) -> tuple[int, float]:
+            temp=None, *, width: int = 0, height: int = 0) -> tuple[int, float]:
     """split::device::kmeans::cluster (cluster.cu:27-93).  Returns
-    (iterations, last shift); host-synchronous like the reference."""
+    (iterations, last shift); host-synchronous like the reference.  With
+    width and height the kernel walks 16x16 tiles of the image."""
     torch = _torch()
     p, n = _points(points)
     c, k = _cent(centroids)
@@ -199,8 +215,16 @@ def cluster(points, centroids, labels, max_iter: int, threshold: float = 1.0,
     if temp is None:
         temp = _temp(L.split_hip_cluster_workspace(n, k, 3), points)
     r = ClusterResult()
-    _check(L.split_hip_cluster(p, n, c, k, lb, temp.data_ptr(), temp.numel(), int(max_iter),
-                               float(threshold), ctypes.byref(r), _stream()), "cluster")
+    if width:
+        if width * height != n:
+            raise ValueError("width*height must equal N")
+        _check(L.split_hip_cluster_image(p, width, height, c, k, lb, temp.data_ptr(),
+                                         temp.numel(), int(max_iter), float(threshold),
+                                         ctypes.byref(r), _stream()), "cluster_image")
+    else:
+        _check(L.split_hip_cluster(p, n, c, k, lb, temp.data_ptr(), temp.numel(),
+                                   int(max_iter), float(threshold), ctypes.byref(r),
+                                   _stream()), "cluster")
     return int(r.iterations), float(r.shift)
 
 

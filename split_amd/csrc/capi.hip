@@ -49,10 +49,10 @@ int device_cu_count() {
 }
 
 // Defined in kmeans.hip / ccl.hip.
-int cluster_async(const float* pts, int64_t n, float* centroids, int k,
-                  int32_t* labels, void* temp, size_t temp_bytes,
-                  int max_iter, float threshold, hipStream_t stream,
-                  Mailbox** mbox_out);
+int cluster_async(const float* pts, int64_t n, int width, int height,
+                  float* centroids, int k, int32_t* labels, void* temp,
+                  size_t temp_bytes, int max_iter, float threshold,
+                  hipStream_t stream, Mailbox** mbox_out);
 int connected_components_impl(const int32_t* labels, int W, int H,
                               int32_t* out, int conn, hipStream_t stream);
 struct CompressState;
@@ -107,8 +107,9 @@ int split_hip_segment(const float* points, int32_t width, int32_t height,
     return fail(SPLIT_HIP_ERR_TEMP, "temp holds %zu bytes, segment needs %zu",
                 temp_bytes, need);
   Mailbox* mb = nullptr;
-  SPLIT_HIP_TRY(cluster_async(points, n, centroids, k, labels, temp,
-                              temp_bytes, max_iter, threshold, stream, &mb));
+  SPLIT_HIP_TRY(cluster_async(points, n, width, height, centroids, k, labels,
+                              temp, temp_bytes, max_iter, threshold, stream,
+                              &mb));
   SPLIT_HIP_TRY(connected_components_impl(labels, width, height, segments,
                                           connectivity, stream));
   if (compress) {

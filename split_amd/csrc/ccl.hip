@@ -204,16 +204,18 @@ __device__ __forceinline__ int wave_min_i(int x) {
   x = min(x, dpp_i<kDppQuadXor2>(x));
   x = min(x, dpp_i<kDppRowHalfMirror>(x));
   x = min(x, dpp_i<kDppRowMirror>(x));
-  return min(min(readlane_i(x, 0), readlane_i(x, 16)),
-             min(readlane_i(x, 32), readlane_i(x, 48)));
+  x = min(x, dpp_i<kDppRowBcast15, 0xA>(x, INT_MAX));
+  x = min(x, dpp_i<kDppRowBcast31, 0xC>(x, INT_MAX));
+  return readlane_i(x, 63);
 }
 __device__ __forceinline__ int wave_max_i(int x) {
   x = max(x, dpp_i<kDppQuadXor1>(x));
   x = max(x, dpp_i<kDppQuadXor2>(x));
   x = max(x, dpp_i<kDppRowHalfMirror>(x));
   x = max(x, dpp_i<kDppRowMirror>(x));
-  return max(max(readlane_i(x, 0), readlane_i(x, 16)),
-             max(readlane_i(x, 32), readlane_i(x, 48)));
+  x = max(x, dpp_i<kDppRowBcast15, 0xA>(x, INT_MIN));
+  x = max(x, dpp_i<kDppRowBcast31, 0xC>(x, INT_MIN));
+  return readlane_i(x, 63);
 }
 
 __global__ __launch_bounds__(256) void compress_minmax(const int32_t* v,

@@ -50,17 +50,19 @@ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 //
 // Reductions run on all 64 lanes (callers keep EXEC full).  Inside each row
 // of 16 lanes: quad xor-1, quad xor-2, half-row mirror, row mirror — after
-// which every lane of a row holds the row result — then the four row results
-// are read with v_readlane and combined in a fixed order, so the result is
-// wave-uniform and bit-identical run to run.
+// which every lane of a row holds the row result.  Then row_bcast:15 folds
+// row 0 into row 1 and row 2 into row 3, row_bcast:31 folds rows 0-1 into
+// rows 2-3, and lane 63 holds the wave result, read with v_readlane.  The
+// combination order is fixed, so float sums are bit-identical run to run.
 
-template <int CTRL>
-__device__ __forceinline__ int dpp_i(int x) {
-  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int dpp_i(int x, int old = 0) {
+  return __builtin_amdgcn_update_dpp(old, x, CTRL, ROW_MASK, 0xF, false);
 }
-template <int CTRL>
-__device__ __forceinline__ float dpp_f(float x) {
-  return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_f(float x, float old = 0.f) {
+  return __int_as_float(
+      dpp_i<CTRL, ROW_MASK>(__float_as_int(x), __float_as_int(old)));
 }
 
 __device__ __forceinline__ float readlane_f(float x, int lane) {
@@ -77,14 +79,19 @@ constexpr int kDppQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]
 constexpr int kDppQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]
 constexpr int kDppRowMirror = 0x140;
 constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
 
 struct OpAdd {
+  static constexpr float identity = 0.f;
   __device__ float operator()(float a, float b) const { return a + b; }
 };
 struct OpMin {
+  static constexpr float identity = INFINITY;
   __device__ float operator()(float a, float b) const { return fminf(a, b); }
 };
 struct OpMax {
+  static constexpr float identity = -INFINITY;
   __device__ float operator()(float a, float b) const { return fmaxf(a, b); }
 };
 
@@ -94,8 +101,9 @@ __device__ __forceinline__ float wave_allreduce(float x, Op op) {
   x = op(x, dpp_f<kDppQuadXor2>(x));
   x = op(x, dpp_f<kDppRowHalfMirror>(x));
   x = op(x, dpp_f<kDppRowMirror>(x));
-  return op(op(readlane_f(x, 0), readlane_f(x, 16)),
-            op(readlane_f(x, 32), readlane_f(x, 48)));
+  x = op(x, dpp_f<kDppRowBcast15, 0xA>(x, Op::identity));
+  x = op(x, dpp_f<kDppRowBcast31, 0xC>(x, Op::identity));
+  return readlane_f(x, 63);
 }
 
 __device__ __forceinline__ unsigned wave_sum_u32(unsigned x) {
@@ -103,8 +111,9 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned x) {
   x += (unsigned)dpp_i<kDppQuadXor2>((int)x);
   x += (unsigned)dpp_i<kDppRowHalfMirror>((int)x);
   x += (unsigned)dpp_i<kDppRowMirror>((int)x);
-  return ((unsigned)readlane_i((int)x, 0) + (unsigned)readlane_i((int)x, 16)) +
-         ((unsigned)readlane_i((int)x, 32) + (unsigned)readlane_i((int)x, 48));
+  x += (unsigned)dpp_i<kDppRowBcast15, 0xA>((int)x, 0);
+  x += (unsigned)dpp_i<kDppRowBcast31, 0xC>((int)x, 0);
+  return (unsigned)readlane_i((int)x, 63);
 }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }

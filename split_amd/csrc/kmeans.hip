@@ -8,12 +8,17 @@
 // iteration.  Here one pass over the three SoA float planes does all of it:
 //
 //  * Each wave owns a contiguous run of 256-pixel segments (64 lanes x 4
-//    pixels, float4 loads per plane when aligned).
-//  * Per segment, the wave reduces the colour box of its pixels, bounds every
-//    centroid's squared distance to that box from below and above, and keeps
-//    only centroids whose lower bound is within 1e-5 (relative) of the
-//    smallest upper bound.  A dropped centroid's float distance is strictly
-//    larger than the winner's for every pixel in the box, so the labels equal
+//    pixels, float4 loads per plane when aligned).  When the image width is
+//    known a segment is a 16x16 tile (lane l: row l/4, columns 4*(l%4)..+3),
+//    whose colours are far more compact than a 1x256 run of a row.
+//  * Per segment, the wave reduces the colour box of its pixels and drops
+//    every centroid that provably cannot be the nearest for any pixel in it:
+//    (a) its squared distance to the box exceeds the smallest farthest-corner
+//    distance, or (b) the whole box lies on the far side of its bisector with
+//    the best-guess centroid g (the one with that smallest corner distance).
+//    Both tests keep a 1e-5 relative margin, far above the few-ulp error of
+//    the float bounds and distances, so a dropped centroid's float distance
+//    is strictly larger than g's for every pixel in the box: the labels equal
 //    the dense float argmin over all k (ties to the lowest index).
 //  * Centroids live in VGPRs (lane j holds centroid j + 64*t); survivors are
 //    broadcast with v_readlane in increasing index order.
@@ -39,8 +44,11 @@ constexpr size_t kStateBytes = 256;   // LloydState slot at the head of temp
 struct StepArgs {
   const float* pts;          // plane c at pts + c*n
   int64_t n;
-  int64_t nseg;              // ceil(n / kSegPx)
+  int64_t nseg;              // segments: ceil(n / kSegPx) or tile count
   int k;
+  int width;                 // > 0: 16x16-tile traversal of width x height
+  int height;
+  int tiles_x;               // ceil(width / 16)
   int use_prev;              // Lloyd label pass: assign against previous
   const float* cent;         // centroids to assign against (standalone)
   float* cent_out;           // new centroids (standalone calculate)
@@ -55,31 +63,60 @@ struct StepArgs {
 
 // ------------------------------------------------------------- loading ---
 
-template <bool VEC>
-__device__ __forceinline__ void load_segment(const StepArgs& a, int64_t base,
+// Traversal modes: 1-D runs of 256 pixels (dword or float4 loads) or 16x16
+// tiles (dword or float4 loads per lane row).
+enum Mode { kRun = 0, kRunVec = 1, kTile = 2, kTileVec = 3 };
+constexpr int kTileSide = 16;
+
+// Loads segment s: x/y/z[i] of pixel idx[i], v[i] whether it exists, lvec
+// whether this lane's four pixels are consecutive and 16-byte aligned (so
+// its labels can be read or written as one int4).
+template <int MODE>
+__device__ __forceinline__ void load_segment(const StepArgs& a, int64_t s,
                                              int lane, float (&x)[4],
                                              float (&y)[4], float (&z)[4],
                                              bool (&v)[4], int64_t (&idx)[4],
-                                             bool& full) {
+                                             bool& lvec) {
   const float* __restrict__ p0 = a.pts;
   const float* __restrict__ p1 = a.pts + a.n;
   const float* __restrict__ p2 = a.pts + 2 * a.n;
-  full = VEC && (base + kSegPx <= a.n);
-  if (full) {
-    const int64_t o = base + 4 * lane;
+  int64_t o;
+  bool all;
+  if constexpr (MODE == kTile || MODE == kTileVec) {
+    const int ty = (int)(s / a.tiles_x);
+    const int tx = (int)(s - (int64_t)ty * a.tiles_x);
+    const int row = ty * kTileSide + (lane >> 2);
+    const int col = tx * kTileSide + (lane & 3) * 4;
+    o = (int64_t)row * a.width + col;
+    const bool rowok = row < a.height;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = rowok && col + i < a.width;
+      idx[i] = o + i;
+    }
+    all = rowok && col + 3 < a.width;
+    lvec = MODE == kTileVec && all;
+  } else {
+    const int64_t base = s * kSegPx;
+    all = base + kSegPx <= a.n;
+    lvec = MODE == kRunVec && all;
+    o = base + 4 * lane;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      idx[i] = lvec ? o + i : base + lane + 64 * i;
+      v[i] = idx[i] < a.n;
+    }
+  }
+  if (lvec) {
     const float4 X = *reinterpret_cast<const float4*>(p0 + o);
     const float4 Y = *reinterpret_cast<const float4*>(p1 + o);
     const float4 Z = *reinterpret_cast<const float4*>(p2 + o);
     x[0] = X.x; x[1] = X.y; x[2] = X.z; x[3] = X.w;
     y[0] = Y.x; y[1] = Y.y; y[2] = Y.z; y[3] = Y.w;
     z[0] = Z.x; z[1] = Z.y; z[2] = Z.z; z[3] = Z.w;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) { v[i] = true; idx[i] = o + i; }
   } else {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      idx[i] = base + lane + 64 * i;
-      v[i] = idx[i] < a.n;
       const int64_t j = v[i] ? idx[i] : 0;
       x[i] = p0[j];
       y[i] = p1[j];
@@ -99,8 +136,9 @@ __device__ __forceinline__ float sqdist(float px, float py, float pz, float cx,
 template <int CPL>
 __device__ __forceinline__ void assign_segment(
     const float (&cx)[CPL], const float (&cy)[CPL], const float (&cz)[CPL],
-    int k, int lane, const float (&x)[4], const float (&y)[4],
-    const float (&z)[4], const bool (&v)[4], int (&lab)[4]) {
+    const float (&cn)[CPL], int k, int lane, const float (&x)[4],
+    const float (&y)[4], const float (&z)[4], const bool (&v)[4],
+    int (&lab)[4]) {
   // Colour box of the segment's valid pixels.
   float lx = INFINITY, ly = INFINITY, lz = INFINITY;
   float hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
@@ -114,8 +152,8 @@ __device__ __forceinline__ void assign_segment(
   ly = wave_allreduce(ly, OpMin()); hy = wave_allreduce(hy, OpMax());
   lz = wave_allreduce(lz, OpMin()); hz = wave_allreduce(hz, OpMax());
 
-  // Lower / upper bounds of each centroid's squared distance to the box.
-  float lower[CPL];
+  // (a) lower / upper bounds of each centroid's squared distance to the box.
+  float lower[CPL], upv[CPL];
   float upper = INFINITY;
 #pragma unroll
   for (int t = 0; t < CPL; ++t) {
@@ -127,25 +165,55 @@ __device__ __forceinline__ void assign_segment(
     const float fy = fmaxf(fabsf(cy[t] - ly), fabsf(cy[t] - hy));
     const float fz = fmaxf(fabsf(cz[t] - lz), fabsf(cz[t] - hz));
     lower[t] = live ? fmaf(ez, ez, fmaf(ey, ey, ex * ex)) : INFINITY;
-    upper = live ? fminf(upper, fmaf(fz, fz, fmaf(fy, fy, fx * fx))) : upper;
+    upv[t] = live ? fmaf(fz, fz, fmaf(fy, fy, fx * fx)) : INFINITY;
+    upper = fminf(upper, upv[t]);
   }
   upper = wave_allreduce(upper, OpMin());
-  // 1e-5 relative margin >> the few-ulp error of the float bounds and
-  // distances, so a dropped centroid can never be (or tie) the float argmin.
   const float keep = upper * 1.00001f;
+
+  // (b) bisector test against g, the lowest-index centroid whose farthest
+  // corner is nearest.  For any pixel p, |p-c_j|^2 - |p-c_g|^2 =
+  // 2 (b_j - p.w_j) with w_j = c_j - c_g and b_j = (|c_j|^2 - |c_g|^2) / 2,
+  // so j is dropped when max over the box of p.w_j stays below b_j by the
+  // margin.
+  int g = 0;
+#pragma unroll
+  for (int t = CPL - 1; t >= 0; --t) {
+    const uint64_t m = ballot(lane + 64 * t < k && upv[t] == upper);
+    if (m) g = 64 * t + __builtin_ctzll(m);
+  }
+  float gx = 0.f, gy = 0.f, gz = 0.f, gn = 0.f;
+#pragma unroll
+  for (int t = 0; t < CPL; ++t) {
+    if ((g >> 6) == t) {
+      gx = readlane_f(cx[t], g & 63);
+      gy = readlane_f(cy[t], g & 63);
+      gz = readlane_f(cz[t], g & 63);
+      gn = readlane_f(cn[t], g & 63);
+    }
+  }
+  const float ax = fmaxf(fabsf(lx), fabsf(hx)), ay = fmaxf(fabsf(ly), fabsf(hy)),
+              az = fmaxf(fabsf(lz), fabsf(hz));
 
   float bd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
   int bk[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int t = 0; t < CPL; ++t) {
-    uint64_t m = ballot(lane + 64 * t < k && lower[t] <= keep);
+    const float wx = cx[t] - gx, wy = cy[t] - gy, wz = cz[t] - gz;
+    const float b = 0.5f * (cn[t] - gn);
+    const float mx = fmaf(wz, wz > 0.f ? hz : lz,
+                          fmaf(wy, wy > 0.f ? hy : ly, wx * (wx > 0.f ? hx : lx)));
+    const float scale = 1.f + fabsf(b) + upper +
+                        fmaf(az, fabsf(wz), fmaf(ay, fabsf(wy), ax * fabsf(wx)));
+    const bool side = mx >= b - 1e-5f * scale;
+    uint64_t m = ballot(lane + 64 * t < k && lower[t] <= keep && side);
     while (m) {
-      const int b = __builtin_ctzll(m);
+      const int bl = __builtin_ctzll(m);
       m &= m - 1;
-      const float ccx = readlane_f(cx[t], b);
-      const float ccy = readlane_f(cy[t], b);
-      const float ccz = readlane_f(cz[t], b);
-      const int j = 64 * t + b;
+      const float ccx = readlane_f(cx[t], bl);
+      const float ccy = readlane_f(cy[t], bl);
+      const float ccz = readlane_f(cz[t], bl);
+      const int j = 64 * t + bl;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float d = sqdist(x[i], y[i], z[i], ccx, ccy, ccz);
@@ -335,7 +403,7 @@ __device__ void finalize_centroids(const StepArgs& a, const float* cent,
 
 // -------------------------------------------------------------- kernel ---
 
-template <int CPL, bool VEC, bool ASSIGN, bool ACCUM>
+template <int CPL, int MODE, bool ASSIGN, bool ACCUM>
 __global__ __launch_bounds__(1024) void lloyd_step(StepArgs a) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x & (kWave - 1);
@@ -355,7 +423,7 @@ __global__ __launch_bounds__(1024) void lloyd_step(StepArgs a) {
     arrive = &a.state->arrive;
   }
 
-  float cx[CPL], cy[CPL], cz[CPL];
+  float cx[CPL], cy[CPL], cz[CPL], cn[CPL];
   if constexpr (ASSIGN) {
 #pragma unroll
     for (int t = 0; t < CPL; ++t) {
@@ -364,6 +432,7 @@ __global__ __launch_bounds__(1024) void lloyd_step(StepArgs a) {
       cx[t] = live ? cent[j] : 0.f;
       cy[t] = live ? cent[k + j] : 0.f;
       cz[t] = live ? cent[2 * k + j] : 0.f;
+      cn[t] = fmaf(cz[t], cz[t], fmaf(cy[t], cy[t], cx[t] * cx[t]));
     }
   }
 
@@ -380,16 +449,15 @@ __global__ __launch_bounds__(1024) void lloyd_step(StepArgs a) {
 
   RunAcc run{0.f, 0.f, 0.f, 0u, -1};
   for (int64_t s = s0; s < s1; ++s) {
-    const int64_t base = s * kSegPx;
     float x[4], y[4], z[4];
-    bool v[4], full;
+    bool v[4], lvec;
     int64_t idx[4];
-    load_segment<VEC>(a, base, lane, x, y, z, v, idx, full);
+    load_segment<MODE>(a, s, lane, x, y, z, v, idx, lvec);
     int lab[4];
     if constexpr (ASSIGN) {
-      assign_segment<CPL>(cx, cy, cz, k, lane, x, y, z, v, lab);
+      assign_segment<CPL>(cx, cy, cz, cn, k, lane, x, y, z, v, lab);
       if (a.labels) {
-        if (full) {
+        if (lvec) {
           *reinterpret_cast<int4*>(a.labels + idx[0]) =
               make_int4(lab[0], lab[1], lab[2], lab[3]);
         } else {
@@ -399,7 +467,7 @@ __global__ __launch_bounds__(1024) void lloyd_step(StepArgs a) {
         }
       }
     } else {
-      if (full) {
+      if (lvec) {
         const int4 L = *reinterpret_cast<const int4*>(a.labels_in + idx[0]);
         lab[0] = L.x; lab[1] = L.y; lab[2] = L.z; lab[3] = L.w;
       } else {
@@ -525,7 +593,7 @@ struct StepCfg {
   int grid;
   int threads;
   size_t lds;
-  bool vec;
+  int mode;
 };
 
 // partial_cap: bytes available for the per-block partial columns (16*k
@@ -553,38 +621,69 @@ static StepCfg step_cfg(const StepArgs& a, bool accum, size_t partial_cap = 0) {
   } else {
     c.lds = 0;
   }
-  c.vec = (a.n % 4 == 0) && aligned16(a.pts) &&
-          (!a.labels || aligned16(a.labels)) &&
-          (!a.labels_in || aligned16(a.labels_in));
+  const bool vec = (a.n % 4 == 0) && aligned16(a.pts) &&
+                   (!a.labels || aligned16(a.labels)) &&
+                   (!a.labels_in || aligned16(a.labels_in));
+  if (a.width > 0)
+    c.mode = (vec && a.width % 4 == 0) ? kTileVec : kTile;
+  else
+    c.mode = vec ? kRunVec : kRun;
   return c;
+}
+
+template <bool ASSIGN, bool ACCUM, int MODE>
+static void launch_mode(const StepArgs& a, const StepCfg& c,
+                        hipStream_t stream) {
+  const dim3 g(c.grid), b(c.threads);
+  switch (ASSIGN ? cpl_for(a.k) : 1) {
+#define SPLIT_LAUNCH(CPL)                                                    \
+  case CPL:                                                                  \
+    hipLaunchKernelGGL((lloyd_step<CPL, MODE, ASSIGN, ACCUM>), g, b, c.lds,  \
+                       stream, a);                                           \
+    break;
+    SPLIT_LAUNCH(1)
+    SPLIT_LAUNCH(2)
+    SPLIT_LAUNCH(4)
+    SPLIT_LAUNCH(8)
+    SPLIT_LAUNCH(16)
+#undef SPLIT_LAUNCH
+  }
 }
 
 template <bool ASSIGN, bool ACCUM>
 static int launch_step(const StepArgs& a, const StepCfg& c,
                        hipStream_t stream) {
-  const dim3 g(c.grid), b(c.threads);
-#define SPLIT_LAUNCH(CPL, VEC) \
-  hipLaunchKernelGGL((lloyd_step<CPL, VEC, ASSIGN, ACCUM>), g, b, c.lds, stream, a)
-  const int cpl = ASSIGN ? cpl_for(a.k) : 1;
-  if (c.vec) {
-    switch (cpl) {
-      case 1: SPLIT_LAUNCH(1, true); break;
-      case 2: SPLIT_LAUNCH(2, true); break;
-      case 4: SPLIT_LAUNCH(4, true); break;
-      case 8: SPLIT_LAUNCH(8, true); break;
-      default: SPLIT_LAUNCH(16, true); break;
-    }
-  } else {
-    switch (cpl) {
-      case 1: SPLIT_LAUNCH(1, false); break;
-      case 2: SPLIT_LAUNCH(2, false); break;
-      case 4: SPLIT_LAUNCH(4, false); break;
-      case 8: SPLIT_LAUNCH(8, false); break;
-      default: SPLIT_LAUNCH(16, false); break;
-    }
+  switch (c.mode) {
+    case kRun: launch_mode<ASSIGN, ACCUM, kRun>(a, c, stream); break;
+    case kRunVec: launch_mode<ASSIGN, ACCUM, kRunVec>(a, c, stream); break;
+    case kTile:
+      if constexpr (ASSIGN) launch_mode<ASSIGN, ACCUM, kTile>(a, c, stream);
+      break;
+    default:
+      if constexpr (ASSIGN) launch_mode<ASSIGN, ACCUM, kTileVec>(a, c, stream);
+      break;
   }
-#undef SPLIT_LAUNCH
   SPLIT_HIP_CHECK_LAUNCH("lloyd_step");
+  return SPLIT_HIP_OK;
+}
+
+// Geometry of a StepArgs: 1-D runs (width == 0) or 16x16 tiles.
+static void set_geometry(StepArgs& a, int64_t n, int width, int height) {
+  a.n = n;
+  if (width > 0) {
+    a.width = width;
+    a.height = height;
+    a.tiles_x = (width + kTileSide - 1) / kTileSide;
+    a.nseg = (int64_t)a.tiles_x * ((height + kTileSide - 1) / kTileSide);
+  } else {
+    a.width = a.height = a.tiles_x = 0;
+    a.nseg = (n + kSegPx - 1) / kSegPx;
+  }
+}
+
+static int check_image(int width, int height) {
+  if (width < 1 || height < 1)
+    return fail(SPLIT_HIP_ERR_ARGUMENT, "image is %d x %d", width, height);
   return SPLIT_HIP_OK;
 }
 
@@ -635,10 +734,10 @@ static int check_points(const float* pts, int64_t n, int k) {
 
 // Internal entry used by the pipeline: Lloyd loop without the final host
 // synchronisation (the mailbox holds the result once the stream drains).
-int cluster_async(const float* pts, int64_t n, float* centroids, int k,
-                  int32_t* labels, void* temp, size_t temp_bytes,
-                  int max_iter, float threshold, hipStream_t stream,
-                  Mailbox** mbox_out) {
+int cluster_async(const float* pts, int64_t n, int width, int height,
+                  float* centroids, int k, int32_t* labels, void* temp,
+                  size_t temp_bytes, int max_iter, float threshold,
+                  hipStream_t stream, Mailbox** mbox_out) {
   SPLIT_HIP_TRY(check_points(pts, n, k));
   if (!centroids) return fail(SPLIT_HIP_ERR_ARGUMENT, "centroids is null");
   if (!temp) return fail(SPLIT_HIP_ERR_ARGUMENT, "temp is null");
@@ -663,8 +762,7 @@ int cluster_async(const float* pts, int64_t n, float* centroids, int k,
 
   StepArgs a{};
   a.pts = pts;
-  a.n = n;
-  a.nseg = (n + kSegPx - 1) / kSegPx;
+  set_geometry(a, n, width, height);
   a.k = k;
   a.cbuf = cbuf;
   a.state = st;
@@ -722,8 +820,25 @@ int split_hip_label_points(const float* centroids, int32_t k,
     return fail(SPLIT_HIP_ERR_ARGUMENT, "centroids or labels is null");
   StepArgs a{};
   a.pts = points;
-  a.n = n;
-  a.nseg = (n + kSegPx - 1) / kSegPx;
+  set_geometry(a, n, 0, 0);
+  a.k = k;
+  a.cent = centroids;
+  a.labels = labels;
+  const StepCfg c = step_cfg(a, false);
+  return launch_step<true, false>(a, c, stream);
+}
+
+int split_hip_label_image(const float* centroids, int32_t k,
+                          const float* points, int32_t width, int32_t height,
+                          int32_t* labels, split_hip_stream_t stream) {
+  SPLIT_HIP_TRY(check_image(width, height));
+  const int64_t n = (int64_t)width * height;
+  SPLIT_HIP_TRY(check_points(points, n, k));
+  if (!centroids || !labels)
+    return fail(SPLIT_HIP_ERR_ARGUMENT, "centroids or labels is null");
+  StepArgs a{};
+  a.pts = points;
+  set_geometry(a, n, width, height);
   a.k = k;
   a.cent = centroids;
   a.labels = labels;
@@ -750,8 +865,7 @@ int split_hip_calculate_centroids(const int32_t* labels, const float* points,
   char* t = static_cast<char*>(temp);
   StepArgs a{};
   a.pts = points;
-  a.n = n;
-  a.nseg = (n + kSegPx - 1) / kSegPx;
+  set_geometry(a, n, 0, 0);
   a.k = k;
   a.labels_in = labels;
   a.cent_out = centroids;
@@ -775,8 +889,28 @@ int split_hip_cluster(const float* points, int64_t n, float* centroids,
                       split_hip_cluster_result* result,
                       split_hip_stream_t stream) {
   Mailbox* mb = nullptr;
-  SPLIT_HIP_TRY(cluster_async(points, n, centroids, k, labels, temp,
+  SPLIT_HIP_TRY(cluster_async(points, n, 0, 0, centroids, k, labels, temp,
                               temp_bytes, max_iter, threshold, stream, &mb));
+  SPLIT_HIP_CHECK(hipStreamSynchronize(stream));
+  if (result) {
+    const volatile Mailbox* v = mb;
+    result->iterations = v->iter;
+    result->shift = v->shift;
+  }
+  return SPLIT_HIP_OK;
+}
+
+int split_hip_cluster_image(const float* points, int32_t width,
+                            int32_t height, float* centroids, int32_t k,
+                            int32_t* labels, void* temp, size_t temp_bytes,
+                            int32_t max_iter, float threshold,
+                            split_hip_cluster_result* result,
+                            split_hip_stream_t stream) {
+  SPLIT_HIP_TRY(check_image(width, height));
+  Mailbox* mb = nullptr;
+  SPLIT_HIP_TRY(cluster_async(points, (int64_t)width * height, width, height,
+                              centroids, k, labels, temp, temp_bytes,
+                              max_iter, threshold, stream, &mb));
   SPLIT_HIP_CHECK(hipStreamSynchronize(stream));
   if (result) {
     const volatile Mailbox* v = mb;

@@ -66,6 +66,40 @@ def test_label_points_matches_reference_argmin(torch_cuda, plain, k):
     assert not bad.any(), f"{bad.sum()} labels differ beyond the near-tie rule"
 
 
+@pytest.mark.parametrize("shape", [(1, 1), (1, 1000), (1000, 1), (17, 33), (37, 100),
+                                   (64, 64), (129, 130), (512, 512)])
+@pytest.mark.parametrize("k", [1, 8, 32, 65, 256, 1000])
+def test_label_image_matches_reference_argmin(torch_cuda, shape, k):
+    torch = torch_cuda
+    h, w = shape
+    say(f"label_image {w}x{h} k={k}")
+    spec = W.ImageSpec(w, h, seed=77 + w + h, sites=max(1, w * h // 2000), shading=True,
+                       noise_sigma=0.01)
+    img = W.make_image(spec)
+    rng = np.random.default_rng(k)
+    cent = rng.random((3, k), dtype=np.float32)
+    L = torch.empty(w * h, dtype=torch.int32, device="cuda")
+    sa.label_points(dev(torch, cent), dev(torch, img), L, width=w, height=h)
+    assert np.array_equal(host(L), oracle.ref_label_points(cent, img))
+
+
+def test_cluster_image_plain_512(torch_cuda, plain):
+    torch = torch_cuda
+    w, img = plain
+    say("cluster_image plain_512 k=8 x20")
+    init = seeded_centroids(img, w.k, w.init_seed)
+    C = dev(torch, init)
+    L = torch.empty(img.shape[1], dtype=torch.int32, device="cuda")
+    it, _ = sa.cluster(dev(torch, img), C, L, 20, -1.0, width=512, height=512)
+    assert it == 20
+    want_c, want_l = oracle.lloyd_f64(img, init, 20)
+    got_c = host(C)
+    assert np.abs(got_c - want_c).max() <= CENT_TOL
+    c19, _ = oracle.lloyd_f64(img, init, 19)
+    _, gap = oracle.label_f64(img, c19.astype(np.float32))
+    assert not ((host(L) != want_l) & (gap >= TIE_REL)).any()
+
+
 @pytest.mark.parametrize("n", [1, 5, 255, 256, 257, 1000, 4097])
 def test_label_points_ragged_sizes(torch_cuda, n):
     torch = torch_cuda
