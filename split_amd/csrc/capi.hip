@@ -53,12 +53,10 @@ int cluster_async(const float* pts, int64_t n, int width, int height,
                   float* centroids, int k, int32_t* labels, void* temp,
                   size_t temp_bytes, int max_iter, float threshold,
                   hipStream_t stream, Mailbox** mbox_out);
-int connected_components_impl(const int32_t* labels, int W, int H,
-                              int32_t* out, int conn, hipStream_t stream);
 struct CompressState;
-int compress_labels_async(int32_t* labels, int64_t n, void* temp,
-                          size_t temp_bytes, hipStream_t stream,
-                          CompressState** st_out);
+int segments_async(const int32_t* labels, int W, int H, int32_t* segments,
+                   int conn, bool compress, void* temp, size_t temp_bytes,
+                   hipStream_t stream, CompressState** st_out);
 int compress_labels_finish(CompressState* st, int32_t* num_unique,
                            hipStream_t stream);
 
@@ -110,12 +108,10 @@ int split_hip_segment(const float* points, int32_t width, int32_t height,
   SPLIT_HIP_TRY(cluster_async(points, n, width, height, centroids, k, labels,
                               temp, temp_bytes, max_iter, threshold, stream,
                               &mb));
-  SPLIT_HIP_TRY(connected_components_impl(labels, width, height, segments,
-                                          connectivity, stream));
+  CompressState* st = nullptr;
+  SPLIT_HIP_TRY(segments_async(labels, width, height, segments, connectivity,
+                               compress != 0, temp, temp_bytes, stream, &st));
   if (compress) {
-    CompressState* st = nullptr;
-    SPLIT_HIP_TRY(
-        compress_labels_async(segments, n, temp, temp_bytes, stream, &st));
     SPLIT_HIP_TRY(compress_labels_finish(st, num_segments, stream));
   } else {
     SPLIT_HIP_CHECK(hipStreamSynchronize(stream));

@@ -38,12 +38,21 @@ __device__ __forceinline__ int ld_rlx(const int* p) {
 }
 
 // ---------------------------------------------------------- LDS forest ---
+//
+// Parents only ever grow (a link puts the smaller root under the larger),
+// so a root is the largest index of its tree.  find() halves paths with
+// plain stores: a non-root's parent is replaced by its grandparent, which is
+// still an ancestor, and unions only ever write roots (atomicMax, retried
+// when the root moved), so both kinds of write are safe together.
 
-__device__ __forceinline__ int lds_find(const volatile int* par, int i) {
+__device__ __forceinline__ int lds_find(volatile int* par, int x) {
   for (;;) {
-    const int p = par[i];
-    if (p == i) return i;
-    i = p;
+    const int p = par[x];
+    if (p == x) return x;
+    const int gp = par[p];
+    if (gp == p) return p;
+    par[x] = gp;
+    x = gp;
   }
 }
 
@@ -59,47 +68,69 @@ __device__ __forceinline__ void lds_union(int* par, int a, int b) {
   }
 }
 
+// One 256-thread block per 32x32 tile.  Thread t handles tile pixels
+// t + 256*j: a wave covers two tile rows per j, one pixel per lane.
+//  1. Horizontal runs: each pixel's parent is the last pixel of its run in
+//     the row (ballot of "same label as right neighbour", then ctz).
+//  2. Vertical (and, for 8-connectivity, diagonal) unions between runs,
+//     skipping a union when the pixel to the left already made it (same run
+//     below, same run above).
+//  3. Each pixel writes the global index of its root.
 template <int CONN>
 __global__ __launch_bounds__(kTileThreads) void ccl_tile(
     const int32_t* __restrict__ labels, int W, int H, int32_t* out) {
   __shared__ int lab[kTilePx];
   __shared__ int par[kTilePx];
   const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
+  const int tw = min(kTile, W - x0);  // tile width inside the image
+  const int th = min(kTile, H - y0);
+  const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll
   for (int j = 0; j < kTilePx / kTileThreads; ++j) {
     const int li = threadIdx.x + kTileThreads * j;
-    const int gx = x0 + (li & (kTile - 1)), gy = y0 + li / kTile;
-    const bool in = gx < W && gy < H;
-    lab[li] = in ? labels[(int64_t)gy * W + gx] : 0;
-    par[li] = li;
+    const int lx = li & (kTile - 1), ly = li / kTile;
+    const bool in = lx < tw && ly < th;
+    lab[li] = in ? labels[(int64_t)(y0 + ly) * W + x0 + lx] : 0;
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kTilePx / kTileThreads; ++j) {
     const int li = threadIdx.x + kTileThreads * j;
     const int lx = li & (kTile - 1), ly = li / kTile;
-    const int gx = x0 + lx, gy = y0 + ly;
-    if (gx >= W || gy >= H) continue;
+    const bool in = lx < tw && ly < th;
+    const bool right = in && lx + 1 < tw && lab[li + 1] == lab[li];
+    const uint64_t m = ballot(right);
+    const uint32_t row = lane < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
+    // bit tw-1 (and above) is clear, so the shifted complement is nonzero
+    const int end = lx + __builtin_ctz(~(row >> lx));
+    par[li] = in ? li - lx + end : li;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kTilePx / kTileThreads; ++j) {
+    const int li = threadIdx.x + kTileThreads * j;
+    const int lx = li & (kTile - 1), ly = li / kTile;
+    if (lx >= tw || ly >= th || ly == 0) continue;
     const int l = lab[li];
-    if (lx > 0 && lab[li - 1] == l) lds_union(par, li, li - 1);
-    if (ly > 0) {
-      if (lab[li - kTile] == l) lds_union(par, li, li - kTile);
-      if (CONN == 8) {
-        if (lx > 0 && lab[li - kTile - 1] == l)
-          lds_union(par, li, li - kTile - 1);
-        if (lx < kTile - 1 && gx + 1 < W && lab[li - kTile + 1] == l)
-          lds_union(par, li, li - kTile + 1);
-      }
+    const int up = li - kTile;
+    const bool left_same = lx > 0 && lab[li - 1] == l;
+    const bool up_same = lab[up] == l;
+    if (up_same && !(left_same && lab[up - 1] == l)) lds_union(par, li, up);
+    if (CONN == 8) {
+      if (lx > 0 && !left_same && !up_same && lab[up - 1] == l)
+        lds_union(par, li, up - 1);
+      if (lx + 1 < tw && !up_same && lab[up + 1] == l)
+        lds_union(par, li, up + 1);
     }
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kTilePx / kTileThreads; ++j) {
     const int li = threadIdx.x + kTileThreads * j;
-    const int gx = x0 + (li & (kTile - 1)), gy = y0 + li / kTile;
-    if (gx >= W || gy >= H) continue;
+    const int lx = li & (kTile - 1), ly = li / kTile;
+    if (lx >= tw || ly >= th) continue;
     const int r = lds_find(par, li);
-    out[(int64_t)gy * W + gx] =
+    out[(int64_t)(y0 + ly) * W + x0 + lx] =
         (y0 + r / kTile) * W + x0 + (r & (kTile - 1));
   }
 }
@@ -128,49 +159,81 @@ __device__ __forceinline__ void g_union(int* par, int a, int b) {
 
 // One block per tile; thread t examines one pixel on the tile's top row,
 // left column or (8-connectivity) right column and unions it with each
-// equal-label neighbour that lies in another tile.
+// equal-label neighbour that lies in another tile.  A union is skipped when
+// a neighbouring edge pixel already joins the same two runs (equal labels
+// along the edge on both sides), which leaves about one union per pair of
+// touching components instead of one per edge pixel.
 template <int CONN>
 __global__ __launch_bounds__(3 * kTile) void ccl_merge(
     const int32_t* __restrict__ labels, int W, int H, int32_t* out) {
   const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
   const int t = threadIdx.x;
-  int gx, gy;
-  if (t < kTile) {            // top row
-    gx = x0 + t; gy = y0;
-  } else if (t < 2 * kTile) { // left column
-    gx = x0; gy = y0 + t - kTile;
-  } else {                    // right column (up-right neighbours)
+  int gx, gy, side;
+  if (t < kTile) {             // top row: up, up-left, up-right
+    gx = x0 + t; gy = y0; side = 0;
+  } else if (t < 2 * kTile) {  // left column: left, up-left
+    gx = x0; gy = y0 + t - kTile; side = 1;
+  } else {                     // right column: up-right into the next tile
     if (CONN != 8) return;
-    gx = x0 + kTile - 1; gy = y0 + t - 2 * kTile;
+    gx = x0 + kTile - 1; gy = y0 + t - 2 * kTile; side = 2;
   }
   if (gx >= W || gy >= H) return;
   const int p = gy * W + gx;
   const int l = labels[p];
-  const int tx = gx / kTile, ty = gy / kTile;
-  // "before" neighbours: left, up, and for 8-connectivity up-left, up-right
-  const int ndx[4] = {-1, 0, -1, 1};
-  const int ndy[4] = {0, -1, -1, -1};
-  const int nn = CONN == 8 ? 4 : 2;
-  for (int q = 0; q < nn; ++q) {
-    const int nx = gx + ndx[q], ny = gy + ndy[q];
-    if (nx < 0 || ny < 0 || nx >= W) continue;
-    if (nx / kTile == tx && ny / kTile == ty) continue;
-    const int np = ny * W + nx;
-    if (labels[np] == l) g_union(out, p, np);
+  if (side == 0) {
+    if (gy == 0) return;
+    const int u = p - W;
+    const bool u_same = labels[u] == l;
+    const bool left_in = gx > x0;
+    const bool left_same = left_in && labels[p - 1] == l;
+    if (u_same && !(left_same && labels[u - 1] == l)) g_union(out, p, u);
+    if (CONN == 8) {
+      if (gx > 0 && !u_same && !left_same && labels[u - 1] == l)
+        g_union(out, p, u - 1);
+      if (gx + 1 < W && !u_same && labels[u + 1] == l) g_union(out, p, u + 1);
+    }
+  } else if (side == 1) {
+    if (gx == 0) return;
+    const bool up_in = gy > y0;
+    const bool lf_same = labels[p - 1] == l;
+    const bool up_same = up_in && labels[p - W] == l;
+    if (lf_same && !(up_same && labels[p - W - 1] == l)) g_union(out, p, p - 1);
+    if (CONN == 8 && up_in && !lf_same && !up_same && labels[p - W - 1] == l)
+      g_union(out, p, p - W - 1);
+  } else {
+    if (gy == y0 || gx + 1 >= W) return;
+    const int ur = p - W + 1;
+    if (labels[ur] == l && labels[p - W] != l && labels[p + 1] != l)
+      g_union(out, p, ur);
   }
 }
 
-__global__ void ccl_flatten(int32_t* out, int64_t n) {
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    const int r0 = out[p];
-    int r = r0;
-    for (;;) {
-      const int q = ld_rlx(out + r);
-      if (q == r) break;
-      r = q;
+// Every pixel follows its chain to the root.  With `roots` non-null, wave w
+// also writes the presence bitmap of roots for its 64 pixels (bit p set iff
+// pixel p is a root), which is the value set compress_labels needs.
+__global__ void ccl_flatten(int32_t* out, int64_t n, uint32_t* roots) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = base + threadIdx.x;
+    bool root = false;
+    if (p < n) {
+      const int r0 = out[p];
+      int r = r0;
+      for (;;) {
+        const int q = out[r];
+        if (q == r) break;
+        r = q;
+      }
+      if (r != r0) out[p] = r;
+      root = r == p;
     }
-    if (r != r0) out[p] = r;
+    const uint64_t m = ballot(root);
+    if (roots && lane == 0 && p < n) {
+      const int64_t w = p >> 5;  // p (lane 0) is a multiple of 64
+      *reinterpret_cast<uint2*>(roots + w) =
+          make_uint2((uint32_t)m, (uint32_t)(m >> 32));
+    }
   }
 }
 
@@ -218,15 +281,40 @@ __device__ __forceinline__ int wave_max_i(int x) {
   return readlane_i(x, 63);
 }
 
+// Labels are read four at a time (int4) when the array is 16-byte aligned;
+// `vec` says so.  Element i of chunk c is labels[4c + i].
+__device__ __forceinline__ void load4(const int32_t* v, int64_t n, int64_t c,
+                                      bool vec, int (&x)[4], bool (&ok)[4]) {
+  const int64_t i0 = 4 * c;
+  if (vec && i0 + 3 < n) {
+    const int4 q = *reinterpret_cast<const int4*>(v + i0);
+    x[0] = q.x; x[1] = q.y; x[2] = q.z; x[3] = q.w;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ok[i] = true;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ok[i] = i0 + i < n;
+      x[i] = ok[i] ? v[i0 + i] : 0;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void compress_minmax(const int32_t* v,
-                                                       int64_t n,
+                                                       int64_t n, int vec,
                                                        CompressState* st) {
   int lo = INT_MAX, hi = INT_MIN;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int x = v[i];
-    lo = min(lo, x);
-    hi = max(hi, x);
+  const int64_t chunks = (n + 3) / 4;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < chunks;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    int x[4];
+    bool ok[4];
+    load4(v, n, c, vec, x, ok);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      lo = ok[i] ? min(lo, x[i]) : lo;
+      hi = ok[i] ? max(hi, x[i]) : hi;
+    }
   }
   lo = wave_min_i(lo);
   hi = wave_max_i(hi);
@@ -249,21 +337,41 @@ __global__ void compress_clear(CompressState* st, unsigned* bm,
     bm[i] = 0u;
 }
 
-__global__ void compress_mark(const int32_t* v, int64_t n,
+__global__ void compress_mark(const int32_t* v, int64_t n, int vec,
                               const CompressState* st, unsigned* bm) {
   if (st->err) return;
   const int lo = st->lo;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+  const int64_t chunks = (n + 3) / 4;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < chunks;
        base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = base + threadIdx.x;
-    const int x = i < n ? v[i] : lo;
-    const int prev = __shfl_up(x, 1);
-    const bool lead = (threadIdx.x & (kWave - 1)) == 0 || prev != x;
-    if (i < n && lead) {
-      const uint32_t d = (uint32_t)((int64_t)x - lo);
-      atomicOr(&bm[d >> 5], 1u << (d & 31));
+    const int64_t c = base + threadIdx.x;
+    int x[4];
+    bool ok[4];
+    load4(v, n, c < chunks ? c : chunks - 1, vec, x, ok);
+    // skip a value equal to the one before it (runs are common)
+    const int prev = __shfl_up(x[3], 1);
+    const bool first_lane = (threadIdx.x & (kWave - 1)) == 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int before = i == 0 ? prev : x[i - 1];
+      const bool lead = (i == 0 && first_lane) || before != x[i];
+      if (c < chunks && ok[i] && lead) {
+        const uint32_t d = (uint32_t)((int64_t)x[i] - lo);
+        atomicOr(&bm[d >> 5], 1u << (d & 31));
+      }
     }
   }
+}
+
+// Value set known to be the roots of a connected-component labelling of n
+// pixels: values lie in [0, n) and the bitmap was written by ccl_flatten.
+__global__ void compress_init_roots(CompressState* st, int64_t n) {
+  st->lo = 0;
+  st->hi = (int)(n - 1);
+  st->arrive = 0u;
+  st->err = 0;
+  st->words = (n + 31) / 32;
+  st->total = 0;
 }
 
 // Per 4096-word chunk: exclusive popcount prefix of each word inside the
@@ -360,7 +468,8 @@ static size_t compress_csum_bytes(int64_t words) {
 }
 
 int connected_components_impl(const int32_t* labels, int W, int H,
-                              int32_t* out, int conn, hipStream_t stream) {
+                              int32_t* out, int conn, hipStream_t stream,
+                              uint32_t* roots) {
   if (!labels || !out)
     return fail(SPLIT_HIP_ERR_ARGUMENT, "labels or out is null");
   if (W < 1 || H < 1)
@@ -389,23 +498,27 @@ int connected_components_impl(const int32_t* labels, int W, int H,
   SPLIT_HIP_CHECK_LAUNCH("ccl_tile/ccl_merge");
   const int64_t n = (int64_t)W * H;
   hipLaunchKernelGGL(ccl_flatten, dim3(grid_for(n, 256, 8)), dim3(256), 0,
-                     stream, out, n);
+                     stream, out, n, roots);
   SPLIT_HIP_CHECK_LAUNCH("ccl_flatten");
   return SPLIT_HIP_OK;
 }
 
-int compress_labels_async(int32_t* labels, int64_t n, void* temp,
-                          size_t temp_bytes, hipStream_t stream,
-                          CompressState** st_out) {
-  if (!labels || !temp)
-    return fail(SPLIT_HIP_ERR_ARGUMENT, "labels or temp is null");
-  if (n < 1) return fail(SPLIT_HIP_ERR_ARGUMENT, "n = %lld < 1", (long long)n);
+// Temp block of compress_labels: [state][chunk sums][bitmap][word prefix].
+struct CompressLayout {
+  CompressState* st;
+  unsigned* csum;
+  unsigned* bm;
+  unsigned* prefix;
+  int64_t cap;  // bitmap words the block holds
+};
+
+static int compress_layout(void* temp, size_t temp_bytes, int64_t n,
+                           CompressLayout* L) {
+  if (!temp) return fail(SPLIT_HIP_ERR_ARGUMENT, "temp is null");
   const size_t min_need = split_hip_compress_labels_workspace(n, 1);
   if (temp_bytes < min_need)
     return fail(SPLIT_HIP_ERR_TEMP, "temp holds %zu bytes, needs >= %zu",
                 temp_bytes, min_need);
-  // Largest bitmap (in 32-bit words) the temp block holds next to its
-  // prefix array and chunk sums.
   auto fits = [&](int64_t w) {
     return kCompressHead + compress_csum_bytes(w) +
                2 * align_up((size_t)w * 4, 16) <=
@@ -419,28 +532,74 @@ int compress_labels_async(int32_t* labels, int64_t n, void* temp,
     else
       hi = mid;
   }
-  const int64_t cap = lo;
   char* t = static_cast<char*>(temp);
-  CompressState* st = reinterpret_cast<CompressState*>(t);
-  unsigned* csum = reinterpret_cast<unsigned*>(t + kCompressHead);
-  unsigned* bm = reinterpret_cast<unsigned*>(t + kCompressHead +
-                                             compress_csum_bytes(cap));
-  unsigned* prefix = bm + align_up((size_t)cap * 4, 16) / 4;
+  L->cap = lo;
+  L->st = reinterpret_cast<CompressState*>(t);
+  L->csum = reinterpret_cast<unsigned*>(t + kCompressHead);
+  L->bm = reinterpret_cast<unsigned*>(t + kCompressHead +
+                                      compress_csum_bytes(lo));
+  L->prefix = L->bm + align_up((size_t)lo * 4, 16) / 4;
+  return SPLIT_HIP_OK;
+}
 
-  hipLaunchKernelGGL(compress_init, dim3(1), dim3(1), 0, stream, st);
-  hipLaunchKernelGGL(compress_minmax, dim3(grid_for(n, 256, 8)), dim3(256), 0,
-                     stream, labels, n, st);
-  hipLaunchKernelGGL(compress_clear, dim3(grid_for(cap, 256, 8)), dim3(256), 0,
-                     stream, st, bm, cap);
-  hipLaunchKernelGGL(compress_mark, dim3(grid_for(n, 256, 8)), dim3(256), 0,
-                     stream, labels, n, st, bm);
-  hipLaunchKernelGGL(compress_scan,
-                     dim3(grid_for(cap, kScanChunk, 1)),
-                     dim3(kScanThreads), 0, stream, st, bm, prefix, csum);
+static int compress_rank(const CompressLayout& L, int32_t* labels, int64_t n,
+                         hipStream_t stream) {
+  hipLaunchKernelGGL(compress_scan, dim3(grid_for(L.cap, kScanChunk, 1)),
+                     dim3(kScanThreads), 0, stream, L.st, L.bm, L.prefix,
+                     L.csum);
   hipLaunchKernelGGL(compress_map, dim3(grid_for(n, 256, 8)), dim3(256), 0,
-                     stream, labels, n, st, bm, prefix, csum);
+                     stream, labels, n, L.st, L.bm, L.prefix, L.csum);
+  SPLIT_HIP_CHECK_LAUNCH("compress scan/map");
+  return SPLIT_HIP_OK;
+}
+
+int compress_labels_async(int32_t* labels, int64_t n, void* temp,
+                          size_t temp_bytes, hipStream_t stream,
+                          CompressState** st_out) {
+  if (!labels) return fail(SPLIT_HIP_ERR_ARGUMENT, "labels is null");
+  if (n < 1) return fail(SPLIT_HIP_ERR_ARGUMENT, "n = %lld < 1", (long long)n);
+  CompressLayout L;
+  SPLIT_HIP_TRY(compress_layout(temp, temp_bytes, n, &L));
+  const int vec = ((uintptr_t)labels & 15u) == 0;
+  const int64_t chunks = (n + 3) / 4;
+  hipLaunchKernelGGL(compress_init, dim3(1), dim3(1), 0, stream, L.st);
+  hipLaunchKernelGGL(compress_minmax, dim3(grid_for(chunks, 256, 8)),
+                     dim3(256), 0, stream, labels, n, vec, L.st);
+  hipLaunchKernelGGL(compress_clear, dim3(grid_for(L.cap, 256, 8)), dim3(256),
+                     0, stream, L.st, L.bm, L.cap);
+  hipLaunchKernelGGL(compress_mark, dim3(grid_for(chunks, 256, 8)), dim3(256),
+                     0, stream, labels, n, vec, L.st, L.bm);
   SPLIT_HIP_CHECK_LAUNCH("compress kernels");
-  *st_out = st;
+  SPLIT_HIP_TRY(compress_rank(L, labels, n, stream));
+  *st_out = L.st;
+  return SPLIT_HIP_OK;
+}
+
+// Connected components of `labels` into `segments`, then their dense ranks,
+// with the root bitmap written by the flatten pass (no min/max, clear or
+// mark pass).
+int segments_async(const int32_t* labels, int W, int H, int32_t* segments,
+                   int conn, bool compress, void* temp, size_t temp_bytes,
+                   hipStream_t stream, CompressState** st_out) {
+  const int64_t n = (int64_t)W * H;
+  if (!compress) {
+    *st_out = nullptr;
+    return connected_components_impl(labels, W, H, segments, conn, stream,
+                                     nullptr);
+  }
+  CompressLayout L;
+  SPLIT_HIP_TRY(compress_layout(temp, temp_bytes, n, &L));
+  const int64_t words = 2 * ((n + 63) / 64);
+  if (L.cap < words)
+    return fail(SPLIT_HIP_ERR_TEMP, "temp holds a bitmap of %lld words, %lld"
+                " needed", (long long)L.cap, (long long)words);
+  SPLIT_HIP_TRY(connected_components_impl(labels, W, H, segments, conn,
+                                          stream, L.bm));
+  hipLaunchKernelGGL(compress_init_roots, dim3(1), dim3(1), 0, stream, L.st,
+                     n);
+  SPLIT_HIP_CHECK_LAUNCH("compress_init_roots");
+  SPLIT_HIP_TRY(compress_rank(L, segments, n, stream));
+  *st_out = L.st;
   return SPLIT_HIP_OK;
 }
 
@@ -473,13 +632,14 @@ int split_hip_connected_components(const int32_t* labels, int32_t width,
                                    int32_t* out, int32_t, int32_t connectivity,
                                    split_hip_stream_t stream) {
   return connected_components_impl(labels, width, height, out, connectivity,
-                                   stream);
+                                   stream, nullptr);
 }
 
 size_t split_hip_compress_labels_workspace(int64_t n, int64_t value_span) {
   if (value_span < 1) value_span = 1;
   (void)n;
-  const int64_t words = (value_span + 31) / 32;
+  // whole 64-bit words, as the flatten pass writes the root bitmap in pairs
+  const int64_t words = 2 * ((value_span + 63) / 64);
   return kCompressHead + compress_csum_bytes(words) +
          2 * align_up((size_t)words * 4, 16);
 }
