@@ -142,4 +142,22 @@ struct Mailbox {
   int reserved;
 };
 
+// Per-thread resources of the Lloyd drivers: a host-mapped mailbox and two
+// events, created on first use and kept for the life of the thread.
+struct DriverRes {
+  Mailbox* host = nullptr;
+  Mailbox* dev = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int device = -1;
+};
+int driver_res(DriverRes** out);
+
+// Bounded tiled Lloyd (lloyd_tiles.hip).
+size_t tiles_workspace(int64_t n, int k);
+bool tiles_supported(int width, int k, int max_iter);
+int cluster_tiles_async(const float* pts, int width, int height,
+                        float* centroids, int k, int32_t* labels, void* temp,
+                        size_t temp_bytes, int max_iter, float threshold,
+                        hipStream_t stream, Mailbox** mbox_out);
+
 }  // namespace splithip

@@ -34,21 +34,15 @@
 #include <mutex>
 
 #include "common.hpp"
+#include "lloyd_device.hpp"
 
 namespace splithip {
 
-constexpr int kSegPx = 256;           // pixels per wave segment
 constexpr int kMaxGrid = 1024;        // upper bound on step-kernel blocks
 constexpr size_t kStateBytes = 256;   // LloydState slot at the head of temp
 
-struct StepArgs {
-  const float* pts;          // plane c at pts + c*n
-  int64_t n;
-  int64_t nseg;              // segments: ceil(n / kSegPx) or tile count
+struct StepArgs : Geometry {
   int k;
-  int width;                 // > 0: 16x16-tile traversal of width x height
-  int height;
-  int tiles_x;               // ceil(width / 16)
   int use_prev;              // Lloyd label pass: assign against previous
   const float* cent;         // centroids to assign against (standalone)
   float* cent_out;           // new centroids (standalone calculate)
@@ -60,172 +54,6 @@ struct StepArgs {
   const int32_t* labels_in;  // labels read instead of assigning
   float* partial;            // [4k][gridDim.x] per-block partial columns
 };
-
-// ------------------------------------------------------------- loading ---
-
-// Traversal modes: 1-D runs of 256 pixels (dword or float4 loads) or 16x16
-// tiles (dword or float4 loads per lane row).
-enum Mode { kRun = 0, kRunVec = 1, kTile = 2, kTileVec = 3 };
-constexpr int kTileSide = 16;
-
-// Loads segment s: x/y/z[i] of pixel idx[i], v[i] whether it exists, lvec
-// whether this lane's four pixels are consecutive and 16-byte aligned (so
-// its labels can be read or written as one int4).
-template <int MODE>
-__device__ __forceinline__ void load_segment(const StepArgs& a, int64_t s,
-                                             int lane, float (&x)[4],
-                                             float (&y)[4], float (&z)[4],
-                                             bool (&v)[4], int64_t (&idx)[4],
-                                             bool& lvec) {
-  const float* __restrict__ p0 = a.pts;
-  const float* __restrict__ p1 = a.pts + a.n;
-  const float* __restrict__ p2 = a.pts + 2 * a.n;
-  int64_t o;
-  bool all;
-  if constexpr (MODE == kTile || MODE == kTileVec) {
-    const int ty = (int)(s / a.tiles_x);
-    const int tx = (int)(s - (int64_t)ty * a.tiles_x);
-    const int row = ty * kTileSide + (lane >> 2);
-    const int col = tx * kTileSide + (lane & 3) * 4;
-    o = (int64_t)row * a.width + col;
-    const bool rowok = row < a.height;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      v[i] = rowok && col + i < a.width;
-      idx[i] = o + i;
-    }
-    all = rowok && col + 3 < a.width;
-    lvec = MODE == kTileVec && all;
-  } else {
-    const int64_t base = s * kSegPx;
-    all = base + kSegPx <= a.n;
-    lvec = MODE == kRunVec && all;
-    o = base + 4 * lane;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      idx[i] = lvec ? o + i : base + lane + 64 * i;
-      v[i] = idx[i] < a.n;
-    }
-  }
-  if (lvec) {
-    const float4 X = *reinterpret_cast<const float4*>(p0 + o);
-    const float4 Y = *reinterpret_cast<const float4*>(p1 + o);
-    const float4 Z = *reinterpret_cast<const float4*>(p2 + o);
-    x[0] = X.x; x[1] = X.y; x[2] = X.z; x[3] = X.w;
-    y[0] = Y.x; y[1] = Y.y; y[2] = Y.z; y[3] = Y.w;
-    z[0] = Z.x; z[1] = Z.y; z[2] = Z.z; z[3] = Z.w;
-  } else {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int64_t j = v[i] ? idx[i] : 0;
-      x[i] = p0[j];
-      y[i] = p1[j];
-      z[i] = p2[j];
-    }
-  }
-}
-
-// ---------------------------------------------------------- assignment ---
-
-__device__ __forceinline__ float sqdist(float px, float py, float pz, float cx,
-                                        float cy, float cz) {
-  const float dx = px - cx, dy = py - cy, dz = pz - cz;
-  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-}
-
-template <int CPL>
-__device__ __forceinline__ void assign_segment(
-    const float (&cx)[CPL], const float (&cy)[CPL], const float (&cz)[CPL],
-    const float (&cn)[CPL], int k, int lane, const float (&x)[4],
-    const float (&y)[4], const float (&z)[4], const bool (&v)[4],
-    int (&lab)[4]) {
-  // Colour box of the segment's valid pixels.
-  float lx = INFINITY, ly = INFINITY, lz = INFINITY;
-  float hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    lx = v[i] ? fminf(lx, x[i]) : lx;  hx = v[i] ? fmaxf(hx, x[i]) : hx;
-    ly = v[i] ? fminf(ly, y[i]) : ly;  hy = v[i] ? fmaxf(hy, y[i]) : hy;
-    lz = v[i] ? fminf(lz, z[i]) : lz;  hz = v[i] ? fmaxf(hz, z[i]) : hz;
-  }
-  lx = wave_allreduce(lx, OpMin()); hx = wave_allreduce(hx, OpMax());
-  ly = wave_allreduce(ly, OpMin()); hy = wave_allreduce(hy, OpMax());
-  lz = wave_allreduce(lz, OpMin()); hz = wave_allreduce(hz, OpMax());
-
-  // (a) lower / upper bounds of each centroid's squared distance to the box.
-  float lower[CPL], upv[CPL];
-  float upper = INFINITY;
-#pragma unroll
-  for (int t = 0; t < CPL; ++t) {
-    const bool live = lane + 64 * t < k;
-    const float ex = fmaxf(fmaxf(lx - cx[t], cx[t] - hx), 0.f);
-    const float ey = fmaxf(fmaxf(ly - cy[t], cy[t] - hy), 0.f);
-    const float ez = fmaxf(fmaxf(lz - cz[t], cz[t] - hz), 0.f);
-    const float fx = fmaxf(fabsf(cx[t] - lx), fabsf(cx[t] - hx));
-    const float fy = fmaxf(fabsf(cy[t] - ly), fabsf(cy[t] - hy));
-    const float fz = fmaxf(fabsf(cz[t] - lz), fabsf(cz[t] - hz));
-    lower[t] = live ? fmaf(ez, ez, fmaf(ey, ey, ex * ex)) : INFINITY;
-    upv[t] = live ? fmaf(fz, fz, fmaf(fy, fy, fx * fx)) : INFINITY;
-    upper = fminf(upper, upv[t]);
-  }
-  upper = wave_allreduce(upper, OpMin());
-  const float keep = upper * 1.00001f;
-
-  // (b) bisector test against g, the lowest-index centroid whose farthest
-  // corner is nearest.  For any pixel p, |p-c_j|^2 - |p-c_g|^2 =
-  // 2 (b_j - p.w_j) with w_j = c_j - c_g and b_j = (|c_j|^2 - |c_g|^2) / 2,
-  // so j is dropped when max over the box of p.w_j stays below b_j by the
-  // margin.
-  int g = 0;
-#pragma unroll
-  for (int t = CPL - 1; t >= 0; --t) {
-    const uint64_t m = ballot(lane + 64 * t < k && upv[t] == upper);
-    if (m) g = 64 * t + __builtin_ctzll(m);
-  }
-  float gx = 0.f, gy = 0.f, gz = 0.f, gn = 0.f;
-#pragma unroll
-  for (int t = 0; t < CPL; ++t) {
-    if ((g >> 6) == t) {
-      gx = readlane_f(cx[t], g & 63);
-      gy = readlane_f(cy[t], g & 63);
-      gz = readlane_f(cz[t], g & 63);
-      gn = readlane_f(cn[t], g & 63);
-    }
-  }
-  const float ax = fmaxf(fabsf(lx), fabsf(hx)), ay = fmaxf(fabsf(ly), fabsf(hy)),
-              az = fmaxf(fabsf(lz), fabsf(hz));
-
-  float bd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-  int bk[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int t = 0; t < CPL; ++t) {
-    const float wx = cx[t] - gx, wy = cy[t] - gy, wz = cz[t] - gz;
-    const float b = 0.5f * (cn[t] - gn);
-    const float mx = fmaf(wz, wz > 0.f ? hz : lz,
-                          fmaf(wy, wy > 0.f ? hy : ly, wx * (wx > 0.f ? hx : lx)));
-    const float scale = 1.f + fabsf(b) + upper +
-                        fmaf(az, fabsf(wz), fmaf(ay, fabsf(wy), ax * fabsf(wx)));
-    const bool side = mx >= b - 1e-5f * scale;
-    uint64_t m = ballot(lane + 64 * t < k && lower[t] <= keep && side);
-    while (m) {
-      const int bl = __builtin_ctzll(m);
-      m &= m - 1;
-      const float ccx = readlane_f(cx[t], bl);
-      const float ccy = readlane_f(cy[t], bl);
-      const float ccz = readlane_f(cz[t], bl);
-      const int j = 64 * t + bl;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float d = sqdist(x[i], y[i], z[i], ccx, ccy, ccz);
-        const bool better = d < bd[i];
-        bd[i] = better ? d : bd[i];
-        bk[i] = better ? j : bk[i];
-      }
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) lab[i] = v[i] ? bk[i] : -1;
-}
 
 // --------------------------------------------------------- accumulation ---
 
@@ -455,7 +283,7 @@ __global__ __launch_bounds__(1024) void lloyd_step(StepArgs a) {
     load_segment<MODE>(a, s, lane, x, y, z, v, idx, lvec);
     int lab[4];
     if constexpr (ASSIGN) {
-      assign_segment<CPL>(cx, cy, cz, cn, k, lane, x, y, z, v, lab);
+      assign_segment<CPL, false>(cx, cy, cz, cn, k, lane, x, y, z, v, lab, nullptr);
       if (a.labels) {
         if (lvec) {
           *reinterpret_cast<int4*>(a.labels + idx[0]) =
@@ -694,16 +522,7 @@ static size_t partial_bytes(int k) {
 }
 static size_t partial_bytes_min(int k) { return 16 * (size_t)k; }
 
-// Per-thread resources of the Lloyd driver: a host-mapped mailbox and two
-// events, created on first use and kept for the life of the thread.
-struct DriverRes {
-  Mailbox* host = nullptr;
-  Mailbox* dev = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  int device = -1;
-};
-
-static int driver_res(DriverRes** out) {
+int driver_res(DriverRes** out) {
   static thread_local DriverRes r;
   int dev = 0;
   SPLIT_HIP_CHECK(hipGetDevice(&dev));
@@ -741,6 +560,11 @@ int cluster_async(const float* pts, int64_t n, int width, int height,
   SPLIT_HIP_TRY(check_points(pts, n, k));
   if (!centroids) return fail(SPLIT_HIP_ERR_ARGUMENT, "centroids is null");
   if (!temp) return fail(SPLIT_HIP_ERR_ARGUMENT, "temp is null");
+  if (tiles_supported(width, k, max_iter) &&
+      temp_bytes >= tiles_workspace(n, k))
+    return cluster_tiles_async(pts, width, height, centroids, k, labels, temp,
+                               temp_bytes, max_iter, threshold, stream,
+                               mbox_out);
   const size_t head = kStateBytes + align_up(6 * (size_t)k * sizeof(float), 256);
   if (temp_bytes < head + partial_bytes_min(k))
     return fail(SPLIT_HIP_ERR_TEMP, "temp holds %zu bytes, cluster needs >= %zu",
@@ -877,10 +701,11 @@ int split_hip_calculate_centroids(const int32_t* labels, const float* points,
   return launch_step<false, true>(a, c, stream);
 }
 
-size_t split_hip_cluster_workspace(int64_t, int32_t k, int32_t) {
+size_t split_hip_cluster_workspace(int64_t n, int32_t k, int32_t) {
   if (k < 1) k = 1;
-  return kStateBytes + align_up(6 * (size_t)k * sizeof(float), 256) +
-         partial_bytes(k);
+  const size_t runs = kStateBytes + align_up(6 * (size_t)k * sizeof(float), 256) +
+                      partial_bytes(k);
+  return std::max(runs, tiles_workspace(n, k));
 }
 
 int split_hip_cluster(const float* points, int64_t n, float* centroids,

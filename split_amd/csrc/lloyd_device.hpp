@@ -1,0 +1,230 @@
+// Device-side pieces shared by the Lloyd kernels: traversal geometry,
+// segment loading (1-D runs or 16x16 tiles) and the exact pruned
+// nearest-centroid assignment.
+#pragma once
+
+#include "common.hpp"
+
+namespace splithip {
+
+constexpr int kSegPx = 256;  // pixels per wave segment (64 lanes x 4)
+
+// What a kernel walks: n pixels of three SoA planes, as 1-D runs of 256
+// pixels (width == 0) or as 16x16 tiles of a width x height image.
+struct Geometry {
+  const float* pts;  // plane c at pts + c*n
+  int64_t n;
+  int64_t nseg;      // segments: ceil(n / kSegPx) or tile count
+  int width;         // > 0: 16x16-tile traversal of width x height
+  int height;
+  int tiles_x;       // ceil(width / 16)
+};
+
+// ------------------------------------------------------------- loading ---
+
+// Traversal modes: 1-D runs of 256 pixels (dword or float4 loads) or 16x16
+// tiles (dword or float4 loads per lane row).
+enum Mode { kRun = 0, kRunVec = 1, kTile = 2, kTileVec = 3 };
+constexpr int kTileSide = 16;
+
+// Pixel indices of segment s for this lane: idx[i] of pixel i, v[i] whether
+// it exists, lvec whether the lane's four pixels are consecutive and
+// 16-byte aligned (so per-pixel arrays can be read or written as one
+// 16-byte vector).
+template <int MODE>
+__device__ __forceinline__ void segment_index(const Geometry& a, int64_t s,
+                                              int lane, int64_t (&idx)[4],
+                                              bool (&v)[4], bool& lvec) {
+  if constexpr (MODE == kTile || MODE == kTileVec) {
+    const int ty = (int)(s / a.tiles_x);
+    const int tx = (int)(s - (int64_t)ty * a.tiles_x);
+    const int row = ty * kTileSide + (lane >> 2);
+    const int col = tx * kTileSide + (lane & 3) * 4;
+    const int64_t o = (int64_t)row * a.width + col;
+    const bool rowok = row < a.height;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = rowok && col + i < a.width;
+      idx[i] = o + i;
+    }
+    lvec = MODE == kTileVec && rowok && col + 3 < a.width;
+  } else {
+    const int64_t base = s * kSegPx;
+    lvec = MODE == kRunVec && base + kSegPx <= a.n;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      idx[i] = lvec ? base + 4 * lane + i : base + lane + 64 * i;
+      v[i] = idx[i] < a.n;
+    }
+  }
+}
+
+__device__ __forceinline__ void load_colours(const Geometry& a,
+                                             const int64_t (&idx)[4],
+                                             const bool (&v)[4], bool lvec,
+                                             float (&x)[4], float (&y)[4],
+                                             float (&z)[4]) {
+  const float* __restrict__ p0 = a.pts;
+  const float* __restrict__ p1 = a.pts + a.n;
+  const float* __restrict__ p2 = a.pts + 2 * a.n;
+  if (lvec) {
+    const int64_t o = idx[0];
+    const float4 X = *reinterpret_cast<const float4*>(p0 + o);
+    const float4 Y = *reinterpret_cast<const float4*>(p1 + o);
+    const float4 Z = *reinterpret_cast<const float4*>(p2 + o);
+    x[0] = X.x; x[1] = X.y; x[2] = X.z; x[3] = X.w;
+    y[0] = Y.x; y[1] = Y.y; y[2] = Y.z; y[3] = Y.w;
+    z[0] = Z.x; z[1] = Z.y; z[2] = Z.z; z[3] = Z.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t j = v[i] ? idx[i] : 0;
+      x[i] = p0[j];
+      y[i] = p1[j];
+      z[i] = p2[j];
+    }
+  }
+}
+
+// Loads segment s: x/y/z[i] of pixel idx[i] (see segment_index).
+template <int MODE>
+__device__ __forceinline__ void load_segment(const Geometry& a, int64_t s,
+                                             int lane, float (&x)[4],
+                                             float (&y)[4], float (&z)[4],
+                                             bool (&v)[4], int64_t (&idx)[4],
+                                             bool& lvec) {
+  segment_index<MODE>(a, s, lane, idx, v, lvec);
+  load_colours(a, idx, v, lvec, x, y, z);
+}
+
+// ---------------------------------------------------------- assignment ---
+
+__device__ __forceinline__ float sqdist(float px, float py, float pz, float cx,
+                                        float cy, float cz) {
+  const float dx = px - cx, dy = py - cy, dz = pz - cz;
+  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+// With TOP2, slack[i] also receives a lower bound on the gap between the
+// distance from pixel i to its second-nearest centroid and to its nearest
+// one (non-squared, rounded down): the second-best evaluated distance, the
+// box lower bound of every box-pruned centroid, and for bisector-pruned
+// centroids |p-c_j|^2 >= |p-c_g|^2 + 2 (b_j - max_box p.w_j) >= best^2 + 2 m.
+template <int CPL, bool TOP2>
+__device__ __forceinline__ void assign_segment(
+    const float (&cx)[CPL], const float (&cy)[CPL], const float (&cz)[CPL],
+    const float (&cn)[CPL], int k, int lane, const float (&x)[4],
+    const float (&y)[4], const float (&z)[4], const bool (&v)[4],
+    int (&lab)[4], float* slack) {
+  // Colour box of the segment's valid pixels.
+  float lx = INFINITY, ly = INFINITY, lz = INFINITY;
+  float hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    lx = v[i] ? fminf(lx, x[i]) : lx;  hx = v[i] ? fmaxf(hx, x[i]) : hx;
+    ly = v[i] ? fminf(ly, y[i]) : ly;  hy = v[i] ? fmaxf(hy, y[i]) : hy;
+    lz = v[i] ? fminf(lz, z[i]) : lz;  hz = v[i] ? fmaxf(hz, z[i]) : hz;
+  }
+  lx = wave_allreduce(lx, OpMin()); hx = wave_allreduce(hx, OpMax());
+  ly = wave_allreduce(ly, OpMin()); hy = wave_allreduce(hy, OpMax());
+  lz = wave_allreduce(lz, OpMin()); hz = wave_allreduce(hz, OpMax());
+
+  // (a) lower / upper bounds of each centroid's squared distance to the box.
+  float lower[CPL], upv[CPL];
+  float upper = INFINITY;
+#pragma unroll
+  for (int t = 0; t < CPL; ++t) {
+    const bool live = lane + 64 * t < k;
+    const float ex = fmaxf(fmaxf(lx - cx[t], cx[t] - hx), 0.f);
+    const float ey = fmaxf(fmaxf(ly - cy[t], cy[t] - hy), 0.f);
+    const float ez = fmaxf(fmaxf(lz - cz[t], cz[t] - hz), 0.f);
+    const float fx = fmaxf(fabsf(cx[t] - lx), fabsf(cx[t] - hx));
+    const float fy = fmaxf(fabsf(cy[t] - ly), fabsf(cy[t] - hy));
+    const float fz = fmaxf(fabsf(cz[t] - lz), fabsf(cz[t] - hz));
+    lower[t] = live ? fmaf(ez, ez, fmaf(ey, ey, ex * ex)) : INFINITY;
+    upv[t] = live ? fmaf(fz, fz, fmaf(fy, fy, fx * fx)) : INFINITY;
+    upper = fminf(upper, upv[t]);
+  }
+  upper = wave_allreduce(upper, OpMin());
+  const float keep = upper * 1.00001f;
+
+  // (b) bisector test against g, the lowest-index centroid whose farthest
+  // corner is nearest.  For any pixel p, |p-c_j|^2 - |p-c_g|^2 =
+  // 2 (b_j - p.w_j) with w_j = c_j - c_g and b_j = (|c_j|^2 - |c_g|^2) / 2,
+  // so j is dropped when max over the box of p.w_j stays below b_j by the
+  // margin.
+  int g = 0;
+#pragma unroll
+  for (int t = CPL - 1; t >= 0; --t) {
+    const uint64_t m = ballot(lane + 64 * t < k && upv[t] == upper);
+    if (m) g = 64 * t + __builtin_ctzll(m);
+  }
+  float gx = 0.f, gy = 0.f, gz = 0.f, gn = 0.f;
+#pragma unroll
+  for (int t = 0; t < CPL; ++t) {
+    if ((g >> 6) == t) {
+      gx = readlane_f(cx[t], g & 63);
+      gy = readlane_f(cy[t], g & 63);
+      gz = readlane_f(cz[t], g & 63);
+      gn = readlane_f(cn[t], g & 63);
+    }
+  }
+  const float ax = fmaxf(fabsf(lx), fabsf(hx)), ay = fmaxf(fabsf(ly), fabsf(hy)),
+              az = fmaxf(fabsf(lz), fabsf(hz));
+
+  float bd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  float bd2[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  int bk[4] = {0, 0, 0, 0};
+  float lbox = INFINITY, mbis = INFINITY;  // per-lane, reduced below
+#pragma unroll
+  for (int t = 0; t < CPL; ++t) {
+    const float wx = cx[t] - gx, wy = cy[t] - gy, wz = cz[t] - gz;
+    const float b = 0.5f * (cn[t] - gn);
+    const float mx = fmaf(wz, wz > 0.f ? hz : lz,
+                          fmaf(wy, wy > 0.f ? hy : ly, wx * (wx > 0.f ? hx : lx)));
+    const float scale = 1.f + fabsf(b) + upper +
+                        fmaf(az, fabsf(wz), fmaf(ay, fabsf(wy), ax * fabsf(wx)));
+    const bool side = mx >= b - 1e-5f * scale;
+    const bool live = lane + 64 * t < k;
+    if constexpr (TOP2) {
+      lbox = (live && lower[t] > keep) ? fminf(lbox, lower[t]) : lbox;
+      mbis = (live && lower[t] <= keep && !side)
+                 ? fminf(mbis, (b - mx) - 1e-6f * scale)
+                 : mbis;
+    }
+    uint64_t m = ballot(live && lower[t] <= keep && side);
+    while (m) {
+      const int bl = __builtin_ctzll(m);
+      m &= m - 1;
+      const float ccx = readlane_f(cx[t], bl);
+      const float ccy = readlane_f(cy[t], bl);
+      const float ccz = readlane_f(cz[t], bl);
+      const int j = 64 * t + bl;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float d = sqdist(x[i], y[i], z[i], ccx, ccy, ccz);
+        const bool better = d < bd[i];
+        if constexpr (TOP2) bd2[i] = better ? bd[i] : fminf(bd2[i], d);
+        bd[i] = better ? d : bd[i];
+        bk[i] = better ? j : bk[i];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) lab[i] = v[i] ? bk[i] : -1;
+  if constexpr (TOP2) {
+    lbox = wave_allreduce(lbox, OpMin());
+    mbis = wave_allreduce(mbis, OpMin());
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      // squared lower bound of the runner-up, then a 2e-5 relative shrink
+      // (and growth of the winner) far above the float rounding involved
+      const float l2 = fminf(fminf(bd2[i], lbox), bd[i] + 2.f * fmaxf(mbis, 0.f));
+      const float s = sqrtf(l2 * (1.f - 2e-5f)) - sqrtf(bd[i] * (1.f + 2e-5f));
+      slack[i] = s > 0.f ? s : 0.f;
+    }
+  }
+}
+
+
+}  // namespace splithip

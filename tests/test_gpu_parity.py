@@ -100,6 +100,59 @@ def test_cluster_image_plain_512(torch_cuda, plain):
     assert not ((host(L) != want_l) & (gap >= TIE_REL)).any()
 
 
+@pytest.mark.parametrize("shape,k,iters", [((512, 512), 32, 40), ((333, 517), 8, 30),
+                                           ((256, 256), 256, 12), ((130, 97), 100, 15)])
+def test_cluster_image_bounded_iterations(torch_cuda, shape, k, iters):
+    """Iterations after the first skip pixels whose label is provably
+    unchanged.  Their labels must still equal a dense assignment: the labels
+    of an `iters` run equal label_image against the centroids of an
+    `iters - 1` run (the trajectory is deterministic), and the centroids stay
+    within 1e-5 of the FP64 trajectory."""
+    torch = torch_cuda
+    h, w = shape
+    say(f"cluster_image bounded {w}x{h} k={k} x{iters}")
+    spec = W.ImageSpec(w, h, seed=5 + w, sites=max(4, w * h // 3000), shading=True,
+                       noise_sigma=0.01)
+    img = W.make_image(spec)
+    init = seeded_centroids(img, k, 900 + k)
+    P = dev(torch, img)
+    runs = {}
+    for n_it in (iters - 1, iters):
+        C = dev(torch, init)
+        L = torch.empty(w * h, dtype=torch.int32, device="cuda")
+        it, _ = sa.cluster(P, C, L, n_it, -1.0, width=w, height=h)
+        assert it == n_it
+        runs[n_it] = (host(C), host(L))
+    c_prev, _ = runs[iters - 1]
+    c_last, l_last = runs[iters]
+    L2 = torch.empty(w * h, dtype=torch.int32, device="cuda")
+    sa.label_points(dev(torch, c_prev), P, L2, width=w, height=h)
+    assert np.array_equal(host(L2), l_last)
+    assert np.array_equal(l_last, oracle.ref_label_points(c_prev, img))
+    # teacher-forced: the centroids are the FP64 means of the GPU's own last
+    # labels
+    want_mean, _ = oracle.centroids_f64(l_last, img, k)
+    assert np.abs(c_last - want_mean).max() <= CENT_TOL
+    if k <= 32:
+        # free-running FP64 trajectory (with many small clusters a single
+        # near-tie flip moves a centroid by more than 1e-5, so only here)
+        want_c, _ = oracle.lloyd_f64(img, init, iters)
+        assert np.abs(c_last - want_c).max() <= CENT_TOL
+
+
+def test_cluster_image_repeatable(torch_cuda, plain):
+    torch = torch_cuda
+    w, img = plain
+    say("cluster_image repeatable")
+    out = []
+    for _ in range(2):
+        C = dev(torch, seeded_centroids(img, 16, 4))
+        L = torch.empty(img.shape[1], dtype=torch.int32, device="cuda")
+        sa.cluster(dev(torch, img), C, L, 25, -1.0, width=512, height=512)
+        out.append((host(C), host(L)))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("n", [1, 5, 255, 256, 257, 1000, 4097])
 def test_label_points_ragged_sizes(torch_cuda, n):
     torch = torch_cuda
@@ -229,14 +282,17 @@ def test_cluster_fixed_iterations_plain_512(torch_cuda, plain):
     assert np.abs(host(C2) - oracle.centroids_f64(lab, img, w.k)[0]).max() <= CENT_TOL
 
 
-def test_cluster_stops_on_threshold(torch_cuda, plain):
+@pytest.mark.parametrize("tiled", [False, True])
+def test_cluster_stops_on_threshold(torch_cuda, plain, tiled):
     torch = torch_cuda
     w, img = plain
-    say("cluster plain_512 to tolerance")
+    say(f"cluster plain_512 to tolerance tiled={tiled}")
     init = seeded_centroids(img, 8, 99)
     C = dev(torch, init)
     L = torch.empty(img.shape[1], dtype=torch.int32, device="cuda")
-    it, shift = sa.cluster(dev(torch, img), C, L, 100, 1e-4)
+    geo = dict(width=512, height=512) if tiled else {}
+    it, shift = sa.cluster(dev(torch, img), C, L, 100, 1e-4, **geo)
+    assert it > 2
     assert 1 <= it <= 100
     assert shift <= 1e-4 or it == 100
     # the FP64 trajectory reaches the same centroids after the same count
