@@ -37,7 +37,128 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--cpu-crop", type=int, default=1024)
+    ap.add_argument("--profile-driver", action="store_true",
+                    help="run the fixed workload the counter profiles use and exit: "
+                         "a 256 MiB device copy (traffic scale reference), then 20 "
+                         "tiled Lloyd iterations, 8-connectivity CC and compress")
     return ap.parse_args()
+
+
+def profile_driver(w):
+    import torch
+    import split_amd as sa
+    from split_amd import workloads as W
+    ref = torch.empty(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+    ref.fill_(1.0)
+    ref.clone()
+    torch.cuda.synchronize()
+    width, height = w.image.width, w.image.height
+    P = torch.from_numpy(W.make_image(w.image)).cuda()
+    C = torch.empty((3, w.k), dtype=torch.float32, device="cuda")
+    L = torch.empty(w.n, dtype=torch.int32, device="cuda")
+    S = torch.empty(w.n, dtype=torch.int32, device="cuda")
+    sa.uniform_random_initialize(P, C, w.init_seed)
+    it, _ = sa.cluster(P, C, L, 20, -1.0, width=width, height=height)
+    sa.connected_components(L.view(height, width), S, connectivity=w.connectivity)
+    n = sa.compress_labels(S)
+    torch.cuda.synchronize()
+    log(f"profile driver: {it} iterations, {n} segments")
+
+
+def batch_bench(args):
+    """Batched segmentation of batch_256 (one graph-replayed launch sequence)
+    versus the same images through 256 separate split_hip_segment calls."""
+    import torch
+    import split_amd as sa
+    from split_amd import workloads as W
+    ws = W.batch_workloads()
+    total = sum(w.n for w in ws)
+    log(f"bench: generating {len(ws)} images, {total / 1e6:.1f} Mpix")
+    torch.cuda.init()
+    ktot = sum(w.k for w in ws)
+    cflat = torch.empty(3 * ktot, dtype=torch.float32, device="cuda")
+    images, off = [], 0
+    for i, w in enumerate(ws):
+        P = torch.from_numpy(W.make_image(w.image)).cuda()
+        C = cflat[off:off + 3 * w.k].view(3, w.k)
+        off += 3 * w.k
+        sa.uniform_random_initialize(P, C, w.init_seed)
+        images.append(dict(points=P, width=w.image.width, height=w.image.height, centroids=C,
+                           labels=torch.empty(w.n, dtype=torch.int32, device="cuda"),
+                           segments=torch.empty(w.n, dtype=torch.int32, device="cuda")))
+        if i % 32 == 31:
+            log(f"bench: {i + 1} images resident")
+    init = cflat.clone()
+    temp = torch.empty(sa.segment_batch_workspace(images), dtype=torch.uint8, device="cuda")
+    stemp = torch.empty(max(sa.segment_workspace(w.image.width, w.image.height, w.k)
+                            for w in ws), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    w0 = ws[0]
+
+    def batched():
+        cflat.copy_(init)
+        return sa.segment_batch(images, max_iter=w0.max_iter, threshold=w0.threshold,
+                                connectivity=w0.connectivity, temp=temp)
+
+    def separate():
+        cflat.copy_(init)
+        return [sa.segment(im["points"], im["width"], im["height"], im["centroids"],
+                           im["labels"], im["segments"], max_iter=w0.max_iter,
+                           threshold=w0.threshold, connectivity=w0.connectivity, temp=stemp)
+                for im in images]
+
+    def timed(fn, steps):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            r = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps, r
+
+    for i in range(args.warmup):
+        rb = batched()
+        log(f"bench: batched warmup {i}: iters min/max "
+            f"{min(r.iterations for r in rb)}/{max(r.iterations for r in rb)}")
+    ms_b, rb = timed(batched, args.steps)
+    log(f"bench: batched {ms_b:.2f} ms/step")
+    separate()
+    ms_s, rs = timed(separate, max(1, args.steps))
+    log(f"bench: separate {ms_s:.2f} ms/step")
+    same = all(a.iterations == b.iterations and a.num_segments == b.num_segments
+               for a, b in zip(rb, rs))
+    value = total / (ms_b / 1e3) / 1e6
+    out = {
+        "metric": "segmented Mpix/s (k-means+CC) & Lloyd iters to tol, warm-start vs plain",
+        "value": round(value, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_b, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {
+            "workload": "batch_256",
+            "images": len(ws),
+            "pixels": total,
+            "k": "per image from {4,8,16,32,64}",
+            "init": "seeded sample (plain, no warm start)",
+            "stop": f"shift<{w0.threshold} or {w0.max_iter} iters",
+            "connectivity": w0.connectivity,
+            "lloyd_iters_mean": round(sum(r.iterations for r in rb) / len(rb), 2),
+            "parallelism": "single GPU",
+        },
+        "separate_launches": {"ms_per_step": round(ms_s, 4),
+                              "value": round(total / (ms_s / 1e3) / 1e6, 3),
+                              "speedup_batched": round(ms_s / ms_b, 3),
+                              "same_iterations_and_segments": same},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def log(msg):
@@ -85,7 +206,13 @@ def main():
     import split_amd as sa
     from split_amd import workloads as W
 
+    if args.workload == "batch_256":
+        batch_bench(args)
+        return
     w = W.WORKLOADS[args.workload]
+    if args.profile_driver:
+        profile_driver(w)
+        return
     width, height, n = w.image.width, w.image.height, w.n
     log(f"bench: generating {w.name} ({width}x{height})")
     img = W.make_image(w.image)

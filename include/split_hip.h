@@ -192,6 +192,50 @@ int split_hip_segment(const float* points, int32_t width, int32_t height,
                       int32_t compress, split_hip_cluster_result* result,
                       int32_t* num_segments, split_hip_stream_t stream);
 
+/* ---- batched segmentation ----------------------------------------------
+ * No counterpart in the reference, whose caller segments one image per
+ * cluster -> connected_components -> compress_labels sequence
+ * (samples/src/main.cu:240-252).  This runs that sequence for many images of
+ * uneven size and k at once; each image's results equal what
+ * split_hip_segment gives for it alone (same centroids, labels, segments,
+ * iteration count and shift). */
+#define SPLIT_HIP_BATCH_MAX_K 64
+
+typedef struct split_hip_image_desc {
+  const float* points;  /* 3 SoA planes of width*height floats */
+  float* centroids;     /* [3][k], in: initial, out: final */
+  int32_t* labels;      /* width*height cluster labels */
+  int32_t* segments;    /* width*height segment labels */
+  int32_t width, height;
+  int32_t k;            /* 1..SPLIT_HIP_BATCH_MAX_K */
+  int32_t reserved;
+} split_hip_image_desc;
+
+typedef struct split_hip_batch_result {
+  int32_t iterations;
+  float shift;
+  int32_t num_segments; /* -1 when compress == 0 */
+  int32_t reserved;
+} split_hip_batch_result;
+
+/* Bytes of temp for a batch: a descriptor table plus, per image, centroid
+ * buffers, sums and the root bitmap (about 0.2 bytes per pixel).  0 for an
+ * invalid batch. */
+size_t split_hip_segment_batch_workspace(const split_hip_image_desc* images,
+                                         int32_t count);
+/* Every image: clustering from its centroids (max_iter / threshold as
+ * split_hip_cluster), connected components of its labels and, with
+ * `compress`, compressed segment labels.  use_graph != 0 captures the
+ * launch sequence into a hipGraph on first use (cached per temp pointer and
+ * batch shape) and replays it.  results (count entries) may be NULL.  The
+ * call returns after the batch finished on `stream`. */
+int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
+                            void* temp, size_t temp_bytes, int32_t max_iter,
+                            float threshold, int32_t connectivity,
+                            int32_t compress, int32_t use_graph,
+                            split_hip_batch_result* results,
+                            split_hip_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

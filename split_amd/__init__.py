@@ -16,6 +16,7 @@ kmeans/*.cuh, ccl/{connected_components,compress_labels}.cuh):
     ccl.connected_components(labels_hw, out, connectivity=4)
     ccl.compress_labels(labels) -> number of distinct labels
     segment(points, width, height, centroids, ...) -> SegmentResult
+    segment_batch([{points, width, height, centroids, labels, segments}, ...])
 
 Layouts: points (3, N) float32 SoA planes, centroids (3, K) float32,
 labels int32 — all contiguous CUDA (HIP) tensors.
@@ -46,6 +47,18 @@ class SplitError(RuntimeError):
 
 class ClusterResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("shift", ctypes.c_float)]
+
+
+class ImageDesc(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("centroids", ctypes.c_void_p),
+                ("labels", ctypes.c_void_p), ("segments", ctypes.c_void_p),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("k", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("shift", ctypes.c_float),
+                ("num_segments", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 _c_void_p = ctypes.c_void_p
@@ -91,6 +104,11 @@ CABI = {
                                          _c_float, _c_i32, _c_i32,
                                          ctypes.POINTER(ClusterResult),
                                          ctypes.POINTER(_c_i32), _c_void_p]),
+    "split_hip_segment_batch_workspace": (_c_size_t, [ctypes.POINTER(ImageDesc), _c_i32]),
+    "split_hip_segment_batch": (ctypes.c_int, [ctypes.POINTER(ImageDesc), _c_i32, _c_void_p,
+                                               _c_size_t, _c_i32, _c_float, _c_i32, _c_i32,
+                                               _c_i32, ctypes.POINTER(BatchResult),
+                                               _c_void_p]),
 }
 
 _lib = None
@@ -315,6 +333,50 @@ def segment(points, width: int, height: int, centroids, labels, segments, *,
                                int(bool(compress)), ctypes.byref(r), ctypes.byref(ns),
                                _stream()), "segment")
     return SegmentResult(int(r.iterations), float(r.shift), int(ns.value))
+
+
+def _batch_descs(images):
+    torch = _torch()
+    descs = (ImageDesc * len(images))()
+    for i, im in enumerate(images):
+        p, n = _points(im["points"])
+        w, h = int(im["width"]), int(im["height"])
+        if n != w * h:
+            raise ValueError(f"image {i}: points must hold width*height pixels")
+        c, k = _cent(im["centroids"])
+        lb = _dev(im["labels"], torch.int32, "labels")
+        sg = _dev(im["segments"], torch.int32, "segments")
+        descs[i] = ImageDesc(p, c, lb, sg, w, h, k, 0)
+    return descs
+
+
+def segment_batch_workspace(images) -> int:
+    """Bytes of temp for segment_batch over `images` (dicts as below)."""
+    descs = _batch_descs(images)
+    return int(lib().split_hip_segment_batch_workspace(descs, len(images)))
+
+
+def segment_batch(images, *, max_iter: int = 100, threshold: float = 1e-4,
+                  connectivity: int = 8, compress: bool = True, use_graph: bool = True,
+                  temp=None) -> list[SegmentResult]:
+    """segment() of every image in one batched, graph-replayed launch sequence.
+
+    images: sequence of dicts with keys points (3, w*h), width, height,
+    centroids (3, k) in/out, labels (w*h,), segments (w*h,); k <= 64.
+    Each image's outputs equal segment() of that image alone."""
+    descs = _batch_descs(images)
+    L = lib()
+    count = len(images)
+    if temp is None:
+        temp = _temp(L.split_hip_segment_batch_workspace(descs, count),
+                     images[0]["points"])
+    res = (BatchResult * count)()
+    _check(L.split_hip_segment_batch(descs, count, temp.data_ptr(), temp.numel(),
+                                     int(max_iter), float(threshold), int(connectivity),
+                                     int(bool(compress)), int(bool(use_graph)), res,
+                                     _stream()), "segment_batch")
+    return [SegmentResult(int(r.iterations), float(r.shift), int(r.num_segments))
+            for r in res]
 
 
 def version() -> str:

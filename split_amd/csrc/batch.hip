@@ -1,0 +1,692 @@
+// Batched segmentation: many images of uneven size and k (<= 64) through
+// one captured hipGraph of persistent launches.
+//
+// The images' 16x16 Lloyd tiles are numbered consecutively (image i owns
+// global tiles [tile0_i, tile0_{i+1})).  Each launch of batch_step is one
+// Lloyd iteration of every image that has not stopped: a persistent grid
+// whose waves each walk a contiguous run of global tiles, switching image
+// (centroids, k, scale) where the run crosses an image boundary.  Sums are
+// exact integers (quantised as in the single-image tiled path), so every
+// image gets exactly the centroids and labels it gets on its own.  The wave
+// that completes an image's last tile of the iteration (per-image tile
+// counter) finalises that image: new centroids, shift, stop flag.  Images
+// finish independently; later launches skip their tiles.
+//
+// After the iterations: the final label pass, connected components
+// (32x32 tiles, all images in one grid), the root bitmap, a per-image scan
+// and the rank map — the same passes as the single-image path.
+//
+// The whole sequence is captured once into a hipGraph per (temp, batch
+// shape, settings) and replayed; per-image buffer pointers live in a
+// descriptor table in temp that is uploaded before each replay.
+// A small autotuner picks the step kernel's block size and blocks per CU
+// by timing one iteration of each candidate on the batch itself, cached per
+// batch signature.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "ccl_device.hpp"
+#include "common.hpp"
+#include "lloyd_device.hpp"
+#include "lloyd_sums.hpp"
+
+namespace splithip {
+
+constexpr int kBatchMaxK = 64;
+
+struct BatchImg {
+  // set by the host
+  const float* pts;
+  int32_t* labels;
+  int32_t* segments;
+  const float* centroids_in;
+  float* centroids_out;
+  int W, H, k, tiles_x;  // 16x16 Lloyd tiles
+  int ctiles_x, pad0;    // 32x32 CC tiles
+  int64_t tile0;         // first global Lloyd tile
+  int64_t ctile0;        // first global CC tile
+  int64_t px0;           // first global pixel (multiple of 64)
+  float* cbuf;           // 2 x 3k
+  long long* totals;     // 4k
+  unsigned* bitmap;      // 2*ceil(n/64) words
+  unsigned* prefix;      // ceil(n/32) words
+  // device state
+  int cur, iter, done, nseg;
+  float shift;
+  unsigned maxabs;
+  int tiles_done, pad1;
+};
+
+struct BatchArgs {
+  BatchImg* img;
+  int count;
+  int64_t tiles;   // total Lloyd tiles
+  int64_t ctiles;  // total CC tiles
+  int64_t px;      // total (padded) pixels
+  int max_iter;
+  float threshold;
+};
+
+// Image owning global index g of the field at byte offset OFF (tile0,
+// ctile0 or px0): the last image whose start is <= g.
+template <int64_t BatchImg::*F>
+__device__ __forceinline__ int find_image(const BatchImg* img, int count,
+                                          int64_t g) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (img[mid].*F <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ Geometry image_geometry(const BatchImg& m) {
+  Geometry g;
+  g.pts = m.pts;
+  g.n = (int64_t)m.W * m.H;
+  g.width = m.W;
+  g.height = m.H;
+  g.tiles_x = m.tiles_x;
+  g.nseg = (int64_t)m.tiles_x * ((m.H + kTileSide - 1) / kTileSide);
+  return g;
+}
+
+__global__ void batch_init(BatchArgs a) {
+  BatchImg& m = a.img[blockIdx.x];
+  const int k = m.k;
+  for (int i = threadIdx.x; i < 3 * k; i += blockDim.x) m.cbuf[i] = m.centroids_in[i];
+  for (int i = threadIdx.x; i < 4 * k; i += blockDim.x) m.totals[i] = 0;
+  if (threadIdx.x == 0) {
+    m.cur = 0;
+    m.iter = 0;
+    m.done = a.max_iter <= 0;
+    m.nseg = -1;
+    m.shift = 0.f;
+    m.maxabs = 0u;
+    m.tiles_done = 0;
+  }
+}
+
+// Largest |v| per image, one wave per Lloyd tile.
+__global__ __launch_bounds__(256) void batch_absmax(BatchArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t s = gw; s < a.tiles; s += nw) {
+    const int i = find_image<&BatchImg::tile0>(a.img, a.count, s);
+    const BatchImg& m = a.img[i];
+    const Geometry g = image_geometry(m);
+    int64_t idx[4];
+    bool v[4], lvec;
+    segment_index<kTileVec>(g, s - m.tile0, lane, idx, v, lvec);
+    lvec = lvec && (g.width % 4 == 0) && ((((uintptr_t)g.pts) & 15u) == 0);
+    float x[4], y[4], z[4];
+    load_colours(g, idx, v, lvec, x, y, z);
+    float mx = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      mx = v[j] ? fmaxf(mx, fmaxf(fabsf(x[j]), fmaxf(fabsf(y[j]), fabsf(z[j])))) : mx;
+    mx = wave_allreduce(mx, OpMax());
+    if (lane == 0) atomicMax(&a.img[i].maxabs, __float_as_uint(mx));
+  }
+}
+
+// Per-wave accumulation slice [4][kBatchMaxK] of int64 and 64 doubles of
+// finalisation scratch live in LDS.
+struct WaveLds {
+  long long slice[4 * kBatchMaxK];
+  double scratch[kWave];
+};
+
+// Flushes the wave's slice for image m into its totals, then counts the
+// wave's tiles; the wave that completes the image's tiles finalises it.
+__device__ void batch_finish_segment(const BatchArgs& a, int i, int ntiles,
+                                     WaveLds& w, int lane) {
+  BatchImg& m = a.img[i];
+  const int k = m.k;
+  for (int v = lane; v < 4 * k; v += kWave) {
+    const long long t = w.slice[v];
+    if (t != 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(&m.totals[v]),
+                (unsigned long long)t);
+    w.slice[v] = 0;
+  }
+  __threadfence();
+  int last = 0;
+  if (lane == 0) {
+    const int img_tiles = (int)(image_geometry(m).nseg);
+    const int before = atomicAdd(&m.tiles_done, ntiles);
+    last = before + ntiles == img_tiles;
+  }
+  last = readfirstlane_i(readlane_i(last, 0));
+  if (!last) return;
+  __threadfence();
+  // this wave finalises image i for this iteration
+  const float fscale = scale_of(m.maxabs);
+  const double inv = 1.0 / (double)fscale;
+  const float* cent = m.cbuf + (size_t)m.cur * 3 * k;
+  float* cent_out = m.cbuf + (size_t)(1 - m.cur) * 3 * k;
+  double d2 = 0.0;
+  if (lane < k) {
+    const long long cnt = __hip_atomic_load(&m.totals[3 * k + lane],
+                                            __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long sum = __hip_atomic_load(&m.totals[c * k + lane],
+                                              __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+      const float nc = cnt > 0 ? (float)((double)sum * inv / (double)cnt) : 0.f;
+      const double d = (double)nc - (double)cent[c * k + lane];
+      d2 += d * d;
+      cent_out[c * k + lane] = nc;
+    }
+  }
+  w.scratch[lane] = d2;
+  for (int v = lane; v < 4 * k; v += kWave) m.totals[v] = 0;
+  __threadfence();
+  if (lane == 0) {
+    double s = 0.0;
+    for (int j = 0; j < kWave; ++j) s += w.scratch[j];
+    const float shift = (float)sqrt(s);
+    const int it = m.iter + 1;
+    m.shift = shift;
+    m.iter = it;
+    m.cur = 1 - m.cur;
+    m.done = !(shift > a.threshold) || it >= a.max_iter;
+    m.tiles_done = 0;
+  }
+}
+
+// One Lloyd iteration of every active image (see the file comment).
+__global__ __launch_bounds__(1024) void batch_step(BatchArgs a) {
+  extern __shared__ WaveLds wlds[];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int nwaves = blockDim.x / kWave;
+  WaveLds& w = wlds[wave];
+  for (int v = lane; v < 4 * kBatchMaxK; v += kWave) w.slice[v] = 0;
+
+  const int64_t gw = (int64_t)blockIdx.x * nwaves + wave;
+  const int64_t W = (int64_t)gridDim.x * nwaves;
+  const int64_t s0 = a.tiles * gw / W;
+  const int64_t s1 = a.tiles * (gw + 1) / W;
+
+  int64_t s = s0;
+  while (s < s1) {
+    const int i = find_image<&BatchImg::tile0>(a.img, a.count, s);
+    const BatchImg& m = a.img[i];
+    const Geometry g = image_geometry(m);
+    const int64_t end = std::min<int64_t>(s1, m.tile0 + g.nseg);
+    if (m.done) {
+      s = end;
+      continue;
+    }
+    const int k = m.k;
+    const float* cent = m.cbuf + (size_t)m.cur * 3 * k;
+    float cx[1], cy[1], cz[1], cn[1];
+    {
+      const bool live = lane < k;
+      cx[0] = live ? cent[lane] : 0.f;
+      cy[0] = live ? cent[k + lane] : 0.f;
+      cz[0] = live ? cent[2 * k + lane] : 0.f;
+      cn[0] = fmaf(cz[0], cz[0], fmaf(cy[0], cy[0], cx[0] * cx[0]));
+    }
+    const float fscale = scale_of(m.maxabs);
+    const bool vec = (g.width % 4 == 0) && ((((uintptr_t)g.pts) & 15u) == 0);
+    IRun run{0, 0, 0, 0, -1, 0};
+    for (int64_t t = s; t < end; ++t) {
+      int64_t idx[4];
+      bool v[4], lvec;
+      segment_index<kTileVec>(g, t - m.tile0, lane, idx, v, lvec);
+      lvec = lvec && vec;
+      float x[4], y[4], z[4];
+      load_colours(g, idx, v, lvec, x, y, z);
+      int lab[4];
+      assign_segment<1, false>(cx, cy, cz, cn, k, lane, x, y, z, v, lab,
+                               nullptr);
+      int qx[4], qy[4], qz[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        qx[j] = (int)(x[j] * fscale);
+        qy[j] = (int)(y[j] * fscale);
+        qz[j] = (int)(z[j] * fscale);
+      }
+      const int l0 = readfirstlane_i(lab[0]);
+      const bool same = lab[0] == l0 && lab[1] == l0 && lab[2] == l0 &&
+                        lab[3] == l0;
+      if (l0 >= 0 && ballot(!same) == 0) {
+        if (l0 != run.label || run.tiles >= kRunFlushTiles) {
+          irun_flush(run, w.slice, k, lane);
+          run.label = l0;
+        }
+        run.x += (qx[0] + qx[1]) + (qx[2] + qx[3]);
+        run.y += (qy[0] + qy[1]) + (qy[2] + qy[3]);
+        run.z += (qz[0] + qz[1]) + (qz[2] + qz[3]);
+        run.n += 4;
+        run.tiles += 1;
+      } else {
+        add_by_label(w.slice, k, lane, lab, qx, qy, qz, 1);
+      }
+    }
+    irun_flush(run, w.slice, k, lane);
+    batch_finish_segment(a, i, (int)(end - s), w, lane);
+    s = end;
+  }
+}
+
+// Labels of the last assignment: every image with at least one iteration
+// is assigned against its previous centroid buffer.
+__global__ __launch_bounds__(256) void batch_labels(BatchArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t s = gw; s < a.tiles; s += nw) {
+    const int i = find_image<&BatchImg::tile0>(a.img, a.count, s);
+    const BatchImg& m = a.img[i];
+    if (m.iter == 0) continue;
+    const Geometry g = image_geometry(m);
+    const int k = m.k;
+    const float* cent = m.cbuf + (size_t)(1 - m.cur) * 3 * k;
+    float cx[1], cy[1], cz[1], cn[1];
+    const bool live = lane < k;
+    cx[0] = live ? cent[lane] : 0.f;
+    cy[0] = live ? cent[k + lane] : 0.f;
+    cz[0] = live ? cent[2 * k + lane] : 0.f;
+    cn[0] = fmaf(cz[0], cz[0], fmaf(cy[0], cy[0], cx[0] * cx[0]));
+    int64_t idx[4];
+    bool v[4], lvec;
+    segment_index<kTileVec>(g, s - m.tile0, lane, idx, v, lvec);
+    lvec = lvec && (g.width % 4 == 0) && ((((uintptr_t)g.pts) & 15u) == 0) &&
+           ((((uintptr_t)m.labels) & 15u) == 0);
+    float x[4], y[4], z[4];
+    load_colours(g, idx, v, lvec, x, y, z);
+    int lab[4];
+    assign_segment<1, false>(cx, cy, cz, cn, k, lane, x, y, z, v, lab, nullptr);
+    if (lvec) {
+      *reinterpret_cast<int4*>(m.labels + idx[0]) =
+          make_int4(lab[0], lab[1], lab[2], lab[3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (v[j]) m.labels[idx[j]] = lab[j];
+    }
+  }
+}
+
+__global__ void batch_finish(BatchArgs a) {
+  BatchImg& m = a.img[blockIdx.x];
+  const float* src = m.cbuf + (size_t)m.cur * 3 * m.k;
+  for (int i = threadIdx.x; i < 3 * m.k; i += blockDim.x) m.centroids_out[i] = src[i];
+}
+
+template <int CONN>
+__global__ __launch_bounds__(kCcThreads) void batch_ccl_tile(BatchArgs a) {
+  __shared__ int lab[kCcTilePx];
+  __shared__ int par[kCcTilePx];
+  for (int64_t c = blockIdx.x; c < a.ctiles; c += gridDim.x) {
+    const int i = find_image<&BatchImg::ctile0>(a.img, a.count, c);
+    const BatchImg& m = a.img[i];
+    const int64_t local = c - m.ctile0;
+    const int ty = (int)(local / m.ctiles_x), tx = (int)(local % m.ctiles_x);
+    ccl_tile_body<CONN>(m.labels, m.W, m.H, m.segments, tx, ty, lab, par);
+    __syncthreads();
+  }
+}
+
+template <int CONN>
+__global__ __launch_bounds__(3 * kCcTile) void batch_ccl_merge(BatchArgs a) {
+  for (int64_t c = blockIdx.x; c < a.ctiles; c += gridDim.x) {
+    const int i = find_image<&BatchImg::ctile0>(a.img, a.count, c);
+    const BatchImg& m = a.img[i];
+    const int64_t local = c - m.ctile0;
+    const int ty = (int)(local / m.ctiles_x), tx = (int)(local % m.ctiles_x);
+    ccl_merge_body<CONN>(m.labels, m.W, m.H, m.segments, tx, ty, threadIdx.x);
+  }
+}
+
+// Flatten + root bitmap over the padded global pixel space (a wave never
+// spans two images, as each image starts at a multiple of 64).
+__global__ __launch_bounds__(256) void batch_flatten(BatchArgs a, int compress) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.px;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = base + threadIdx.x;
+    const int64_t wave0 = p - lane;
+    const int i = find_image<&BatchImg::px0>(a.img, a.count, wave0);
+    const BatchImg& m = a.img[i];
+    const int64_t n = (int64_t)m.W * m.H;
+    const int64_t q = p - m.px0;
+    const bool root = q < n && ccl_flatten_one(m.segments, q) == q;
+    const uint64_t bits = ballot(root);
+    if (compress && lane == 0 && wave0 - m.px0 < n) {
+      *reinterpret_cast<uint2*>(m.bitmap + ((wave0 - m.px0) >> 5)) =
+          make_uint2((uint32_t)bits, (uint32_t)(bits >> 32));
+    }
+  }
+}
+
+// Exclusive popcount prefix of each image's root bitmap, one block per image.
+__global__ __launch_bounds__(1024) void batch_scan(BatchArgs a) {
+  BatchImg& m = a.img[blockIdx.x];
+  const int64_t n = (int64_t)m.W * m.H;
+  const int64_t words = (n + 31) / 32;
+  const int nt = blockDim.x;
+  const int64_t per = (words + nt - 1) / nt;
+  const int64_t w0 = (int64_t)threadIdx.x * per;
+  unsigned sum = 0;
+  for (int64_t w = w0; w < w0 + per && w < words; ++w) sum += __popc(m.bitmap[w]);
+  __shared__ unsigned s[1024];
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {
+    const unsigned add = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += add;
+    __syncthreads();
+  }
+  unsigned run = s[threadIdx.x] - sum;
+  for (int64_t w = w0; w < w0 + per && w < words; ++w) {
+    m.prefix[w] = run;
+    run += __popc(m.bitmap[w]);
+  }
+  if (threadIdx.x == nt - 1) m.nseg = (int)s[nt - 1];
+}
+
+__global__ __launch_bounds__(256) void batch_map(BatchArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.px;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = base + threadIdx.x;
+    const int i = find_image<&BatchImg::px0>(a.img, a.count, p - lane);
+    const BatchImg& m = a.img[i];
+    const int64_t q = p - m.px0;
+    if (q < (int64_t)m.W * m.H) {
+      const uint32_t v = (uint32_t)m.segments[q];
+      const uint32_t wd = v >> 5;
+      m.segments[q] =
+          (int32_t)(m.prefix[wd] + __popc(m.bitmap[wd] & ((1u << (v & 31)) - 1u)));
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host ---
+
+struct BatchPlan {
+  std::vector<BatchImg> img;
+  size_t table_bytes = 0, total_bytes = 0;
+  int64_t tiles = 0, ctiles = 0, px = 0;
+};
+
+static int make_plan(const split_hip_image_desc* d, int count, void* temp,
+                     BatchPlan* P) {
+  P->img.assign(count, BatchImg{});
+  char* t = static_cast<char*>(temp);
+  size_t o = align_up((size_t)count * sizeof(BatchImg), 256);
+  P->table_bytes = o;
+  int64_t tiles = 0, ctiles = 0, px = 0;
+  for (int i = 0; i < count; ++i) {
+    BatchImg& m = P->img[i];
+    const split_hip_image_desc& e = d[i];
+    if (e.width < 1 || e.height < 1 || (int64_t)e.width * e.height > INT32_MAX)
+      return fail(SPLIT_HIP_ERR_ARGUMENT, "image %d is %d x %d", i, e.width,
+                  e.height);
+    if (e.k < 1 || e.k > kBatchMaxK)
+      return fail(SPLIT_HIP_ERR_UNSUPPORTED, "image %d: k = %d (batch k <= %d)",
+                  i, e.k, kBatchMaxK);
+    if (t && (!e.points || !e.centroids || !e.labels || !e.segments))
+      return fail(SPLIT_HIP_ERR_ARGUMENT, "image %d has a null buffer", i);
+    const int64_t n = (int64_t)e.width * e.height;
+    m.pts = e.points;
+    m.labels = e.labels;
+    m.segments = e.segments;
+    m.centroids_in = e.centroids;
+    m.centroids_out = e.centroids;
+    m.W = e.width;
+    m.H = e.height;
+    m.k = e.k;
+    m.tiles_x = (e.width + kTileSide - 1) / kTileSide;
+    m.ctiles_x = (e.width + kCcTile - 1) / kCcTile;
+    m.tile0 = tiles;
+    m.ctile0 = ctiles;
+    m.px0 = px;
+    tiles += (int64_t)m.tiles_x * ((e.height + kTileSide - 1) / kTileSide);
+    ctiles += (int64_t)m.ctiles_x * ((e.height + kCcTile - 1) / kCcTile);
+    px += (n + 63) / 64 * 64;
+    auto take = [&](size_t bytes) {
+      char* p = t ? t + o : nullptr;
+      o += align_up(bytes, 256);
+      return p;
+    };
+    m.cbuf = reinterpret_cast<float*>(take(6 * (size_t)e.k * sizeof(float)));
+    m.totals = reinterpret_cast<long long*>(take(4 * (size_t)e.k * 8));
+    m.bitmap = reinterpret_cast<unsigned*>(take(2 * (size_t)((n + 63) / 64) * 4));
+    m.prefix = reinterpret_cast<unsigned*>(take((size_t)((n + 31) / 32) * 4));
+  }
+  P->tiles = tiles;
+  P->ctiles = ctiles;
+  P->px = px;
+  P->total_bytes = o;
+  return SPLIT_HIP_OK;
+}
+
+struct StepTune {
+  int threads = 1024;
+  int per_cu = 1;
+};
+
+static size_t step_lds(int threads) {
+  return (size_t)(threads / kWave) * sizeof(WaveLds);
+}
+
+static void launch_batch_step(const BatchArgs& a, const StepTune& t,
+                              hipStream_t stream) {
+  const int nwaves = t.threads / kWave;
+  const int64_t want = (a.tiles + nwaves - 1) / nwaves;
+  const int grid = (int)std::max<int64_t>(
+      1, std::min<int64_t>(want, (int64_t)device_cu_count() * t.per_cu));
+  hipLaunchKernelGGL(batch_step, dim3(grid), dim3(t.threads), step_lds(t.threads),
+                     stream, a);
+}
+
+// Autotune cache: batch signature -> best step configuration.
+using TuneKey = std::tuple<int, int, int64_t, int64_t, int>;
+static std::mutex g_tune_mu;
+static std::map<TuneKey, StepTune> g_tune;
+
+static int autotune_step(const BatchArgs& a, const BatchPlan& P, int device,
+                         hipStream_t stream, StepTune* out) {
+  int kmax = 0;
+  for (const auto& m : P.img) kmax = std::max(kmax, m.k);
+  const TuneKey key{device, a.count, a.tiles, a.px, kmax};
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      *out = it->second;
+      return SPLIT_HIP_OK;
+    }
+  }
+  hipEvent_t e0, e1;
+  SPLIT_HIP_CHECK(hipEventCreate(&e0));
+  SPLIT_HIP_CHECK(hipEventCreate(&e1));
+  StepTune best;
+  float best_ms = 1e30f;
+  BatchArgs ta = a;
+  ta.max_iter = 1 << 30;  // never stop while timing
+  ta.threshold = -1.f;
+  for (int threads : {256, 512, 1024}) {
+    int occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occ, (const void*)batch_step, threads, step_lds(threads)) !=
+            hipSuccess ||
+        occ < 1)
+      occ = 1;
+    for (int per_cu = 1; per_cu <= occ; per_cu *= 2) {
+      const StepTune c{threads, per_cu};
+      hipLaunchKernelGGL(batch_init, dim3(a.count), dim3(256), 0, stream, ta);
+      hipLaunchKernelGGL(batch_absmax, dim3(8 * device_cu_count()), dim3(256),
+                         0, stream, ta);
+      launch_batch_step(ta, c, stream);  // warm-up
+      SPLIT_HIP_CHECK(hipEventRecord(e0, stream));
+      for (int r = 0; r < 3; ++r) launch_batch_step(ta, c, stream);
+      SPLIT_HIP_CHECK(hipEventRecord(e1, stream));
+      SPLIT_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SPLIT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  }
+  SPLIT_HIP_CHECK_LAUNCH("batch autotune");
+  SPLIT_HIP_CHECK(hipEventDestroy(e0));
+  SPLIT_HIP_CHECK(hipEventDestroy(e1));
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[key] = best;
+  }
+  *out = best;
+  return SPLIT_HIP_OK;
+}
+
+// Enqueues the whole batched segmentation on `stream`.
+static void enqueue_batch(const BatchArgs& a, const StepTune& t, int conn,
+                          int compress, hipStream_t stream) {
+  const int cus = device_cu_count();
+  hipLaunchKernelGGL(batch_init, dim3(a.count), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(batch_absmax, dim3(8 * cus), dim3(256), 0, stream, a);
+  for (int it = 0; it < a.max_iter; ++it) launch_batch_step(a, t, stream);
+  hipLaunchKernelGGL(batch_labels, dim3(8 * cus), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(batch_finish, dim3(a.count), dim3(256), 0, stream, a);
+  const int cgrid = (int)std::min<int64_t>(a.ctiles, (int64_t)cus * 16);
+  if (conn == 8) {
+    hipLaunchKernelGGL(batch_ccl_tile<8>, dim3(cgrid), dim3(kCcThreads), 0,
+                       stream, a);
+    hipLaunchKernelGGL(batch_ccl_merge<8>, dim3(cgrid), dim3(3 * kCcTile), 0,
+                       stream, a);
+  } else {
+    hipLaunchKernelGGL(batch_ccl_tile<4>, dim3(cgrid), dim3(kCcThreads), 0,
+                       stream, a);
+    hipLaunchKernelGGL(batch_ccl_merge<4>, dim3(cgrid), dim3(3 * kCcTile), 0,
+                       stream, a);
+  }
+  hipLaunchKernelGGL(batch_flatten, dim3(8 * cus), dim3(256), 0, stream, a,
+                     compress);
+  if (compress) {
+    hipLaunchKernelGGL(batch_scan, dim3(a.count), dim3(1024), 0, stream, a);
+    hipLaunchKernelGGL(batch_map, dim3(8 * cus), dim3(256), 0, stream, a);
+  }
+}
+
+struct GraphEntry {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+using GraphKey = std::tuple<int, void*, int, int64_t, int64_t, int, float, int,
+                            int, int, int>;
+static std::mutex g_graph_mu;
+static std::map<GraphKey, GraphEntry> g_graphs;
+
+}  // namespace splithip
+
+using namespace splithip;
+
+extern "C" {
+
+size_t split_hip_segment_batch_workspace(const split_hip_image_desc* images,
+                                         int32_t count) {
+  if (!images || count < 1) return 0;
+  BatchPlan P;
+  if (make_plan(images, count, nullptr, &P) != SPLIT_HIP_OK) return 0;
+  return P.total_bytes;
+}
+
+int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
+                            void* temp, size_t temp_bytes, int32_t max_iter,
+                            float threshold, int32_t connectivity,
+                            int32_t compress, int32_t use_graph,
+                            split_hip_batch_result* results,
+                            split_hip_stream_t stream) {
+  if (!images || count < 1 || !temp)
+    return fail(SPLIT_HIP_ERR_ARGUMENT, "images, count or temp missing");
+  if (connectivity != 4 && connectivity != 8)
+    return fail(SPLIT_HIP_ERR_ARGUMENT, "connectivity %d", connectivity);
+  BatchPlan P;
+  SPLIT_HIP_TRY(make_plan(images, count, temp, &P));
+  if (temp_bytes < P.total_bytes)
+    return fail(SPLIT_HIP_ERR_TEMP, "temp holds %zu bytes, batch needs %zu",
+                temp_bytes, P.total_bytes);
+  BatchArgs a;
+  a.img = static_cast<BatchImg*>(temp);
+  a.count = count;
+  a.tiles = P.tiles;
+  a.ctiles = P.ctiles;
+  a.px = P.px;
+  a.max_iter = std::max(0, max_iter);
+  a.threshold = threshold;
+  SPLIT_HIP_CHECK(hipMemcpyAsync(temp, P.img.data(),
+                                 (size_t)count * sizeof(BatchImg),
+                                 hipMemcpyHostToDevice, stream));
+  int device = 0;
+  SPLIT_HIP_CHECK(hipGetDevice(&device));
+  StepTune tune;
+  SPLIT_HIP_TRY(autotune_step(a, P, device, stream, &tune));
+
+  if (use_graph) {
+    const GraphKey key{device, temp, count, P.tiles, P.px, a.max_iter,
+                       threshold, connectivity, compress != 0, tune.threads,
+                       tune.per_cu};
+    hipGraphExec_t exec = nullptr;
+    {
+      std::lock_guard<std::mutex> lock(g_graph_mu);
+      auto it = g_graphs.find(key);
+      if (it != g_graphs.end()) exec = it->second.exec;
+    }
+    if (!exec) {
+      hipStream_t cap = nullptr;
+      SPLIT_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      GraphEntry e;
+      SPLIT_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+      enqueue_batch(a, tune, connectivity, compress != 0, cap);
+      const hipError_t le = hipGetLastError();
+      SPLIT_HIP_CHECK(hipStreamEndCapture(cap, &e.graph));
+      if (le != hipSuccess)
+        return fail(SPLIT_HIP_ERR_HIP, "capture of the batch failed: %s",
+                    hipGetErrorString(le));
+      SPLIT_HIP_CHECK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+      SPLIT_HIP_CHECK(hipStreamDestroy(cap));
+      std::lock_guard<std::mutex> lock(g_graph_mu);
+      g_graphs[key] = e;
+      exec = e.exec;
+    }
+    SPLIT_HIP_CHECK(hipGraphLaunch(exec, stream));
+  } else {
+    enqueue_batch(a, tune, connectivity, compress != 0, stream);
+    SPLIT_HIP_CHECK_LAUNCH("batch kernels");
+  }
+  if (results) {
+    std::vector<BatchImg> back(count);
+    SPLIT_HIP_CHECK(hipMemcpyAsync(back.data(), temp,
+                                   (size_t)count * sizeof(BatchImg),
+                                   hipMemcpyDeviceToHost, stream));
+    SPLIT_HIP_CHECK(hipStreamSynchronize(stream));
+    for (int i = 0; i < count; ++i) {
+      results[i].iterations = back[i].iter;
+      results[i].shift = back[i].shift;
+      results[i].num_segments = compress ? back[i].nseg : -1;
+    }
+  } else {
+    SPLIT_HIP_CHECK(hipStreamSynchronize(stream));
+  }
+  return SPLIT_HIP_OK;
+}
+
+}  // extern "C"

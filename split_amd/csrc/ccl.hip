@@ -26,186 +26,23 @@
 #include <limits.h>
 
 #include "common.hpp"
+#include "ccl_device.hpp"
 
 namespace splithip {
 
-constexpr int kTile = 32;
-constexpr int kTilePx = kTile * kTile;
-constexpr int kTileThreads = 256;
 
-__device__ __forceinline__ int ld_rlx(const int* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---------------------------------------------------------- LDS forest ---
-//
-// Parents only ever grow (a link puts the smaller root under the larger),
-// so a root is the largest index of its tree.  find() halves paths with
-// plain stores: a non-root's parent is replaced by its grandparent, which is
-// still an ancestor, and unions only ever write roots (atomicMax, retried
-// when the root moved), so both kinds of write are safe together.
-
-__device__ __forceinline__ int lds_find(volatile int* par, int x) {
-  for (;;) {
-    const int p = par[x];
-    if (p == x) return x;
-    const int gp = par[p];
-    if (gp == p) return p;
-    par[x] = gp;
-    x = gp;
-  }
-}
-
-__device__ __forceinline__ void lds_union(int* par, int a, int b) {
-  for (;;) {
-    a = lds_find(par, a);
-    b = lds_find(par, b);
-    if (a == b) return;
-    if (a > b) { const int t = a; a = b; b = t; }
-    const int old = atomicMax(&par[a], b);
-    if (old == a) return;
-    a = old;
-  }
-}
-
-// One 256-thread block per 32x32 tile.  Thread t handles tile pixels
-// t + 256*j: a wave covers two tile rows per j, one pixel per lane.
-//  1. Horizontal runs: each pixel's parent is the last pixel of its run in
-//     the row (ballot of "same label as right neighbour", then ctz).
-//  2. Vertical (and, for 8-connectivity, diagonal) unions between runs,
-//     skipping a union when the pixel to the left already made it (same run
-//     below, same run above).
-//  3. Each pixel writes the global index of its root.
 template <int CONN>
-__global__ __launch_bounds__(kTileThreads) void ccl_tile(
+__global__ __launch_bounds__(kCcThreads) void ccl_tile(
     const int32_t* __restrict__ labels, int W, int H, int32_t* out) {
-  __shared__ int lab[kTilePx];
-  __shared__ int par[kTilePx];
-  const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
-  const int tw = min(kTile, W - x0);  // tile width inside the image
-  const int th = min(kTile, H - y0);
-  const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int j = 0; j < kTilePx / kTileThreads; ++j) {
-    const int li = threadIdx.x + kTileThreads * j;
-    const int lx = li & (kTile - 1), ly = li / kTile;
-    const bool in = lx < tw && ly < th;
-    lab[li] = in ? labels[(int64_t)(y0 + ly) * W + x0 + lx] : 0;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kTilePx / kTileThreads; ++j) {
-    const int li = threadIdx.x + kTileThreads * j;
-    const int lx = li & (kTile - 1), ly = li / kTile;
-    const bool in = lx < tw && ly < th;
-    const bool right = in && lx + 1 < tw && lab[li + 1] == lab[li];
-    const uint64_t m = ballot(right);
-    const uint32_t row = lane < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
-    // bit tw-1 (and above) is clear, so the shifted complement is nonzero
-    const int end = lx + __builtin_ctz(~(row >> lx));
-    par[li] = in ? li - lx + end : li;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kTilePx / kTileThreads; ++j) {
-    const int li = threadIdx.x + kTileThreads * j;
-    const int lx = li & (kTile - 1), ly = li / kTile;
-    if (lx >= tw || ly >= th || ly == 0) continue;
-    const int l = lab[li];
-    const int up = li - kTile;
-    const bool left_same = lx > 0 && lab[li - 1] == l;
-    const bool up_same = lab[up] == l;
-    if (up_same && !(left_same && lab[up - 1] == l)) lds_union(par, li, up);
-    if (CONN == 8) {
-      if (lx > 0 && !left_same && !up_same && lab[up - 1] == l)
-        lds_union(par, li, up - 1);
-      if (lx + 1 < tw && !up_same && lab[up + 1] == l)
-        lds_union(par, li, up + 1);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kTilePx / kTileThreads; ++j) {
-    const int li = threadIdx.x + kTileThreads * j;
-    const int lx = li & (kTile - 1), ly = li / kTile;
-    if (lx >= tw || ly >= th) continue;
-    const int r = lds_find(par, li);
-    out[(int64_t)(y0 + ly) * W + x0 + lx] =
-        (y0 + r / kTile) * W + x0 + (r & (kTile - 1));
-  }
+  __shared__ int lab[kCcTilePx];
+  __shared__ int par[kCcTilePx];
+  ccl_tile_body<CONN>(labels, W, H, out, blockIdx.x, blockIdx.y, lab, par);
 }
 
-// ------------------------------------------------------- global forest ---
-
-__device__ __forceinline__ int g_find(const int* par, int i) {
-  for (;;) {
-    const int p = ld_rlx(par + i);
-    if (p == i) return i;
-    i = p;
-  }
-}
-
-__device__ __forceinline__ void g_union(int* par, int a, int b) {
-  for (;;) {
-    a = g_find(par, a);
-    b = g_find(par, b);
-    if (a == b) return;
-    if (a > b) { const int t = a; a = b; b = t; }
-    const int old = atomicMax(par + a, b);
-    if (old == a) return;
-    a = old;
-  }
-}
-
-// One block per tile; thread t examines one pixel on the tile's top row,
-// left column or (8-connectivity) right column and unions it with each
-// equal-label neighbour that lies in another tile.  A union is skipped when
-// a neighbouring edge pixel already joins the same two runs (equal labels
-// along the edge on both sides), which leaves about one union per pair of
-// touching components instead of one per edge pixel.
 template <int CONN>
-__global__ __launch_bounds__(3 * kTile) void ccl_merge(
+__global__ __launch_bounds__(3 * kCcTile) void ccl_merge(
     const int32_t* __restrict__ labels, int W, int H, int32_t* out) {
-  const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
-  const int t = threadIdx.x;
-  int gx, gy, side;
-  if (t < kTile) {             // top row: up, up-left, up-right
-    gx = x0 + t; gy = y0; side = 0;
-  } else if (t < 2 * kTile) {  // left column: left, up-left
-    gx = x0; gy = y0 + t - kTile; side = 1;
-  } else {                     // right column: up-right into the next tile
-    if (CONN != 8) return;
-    gx = x0 + kTile - 1; gy = y0 + t - 2 * kTile; side = 2;
-  }
-  if (gx >= W || gy >= H) return;
-  const int p = gy * W + gx;
-  const int l = labels[p];
-  if (side == 0) {
-    if (gy == 0) return;
-    const int u = p - W;
-    const bool u_same = labels[u] == l;
-    const bool left_in = gx > x0;
-    const bool left_same = left_in && labels[p - 1] == l;
-    if (u_same && !(left_same && labels[u - 1] == l)) g_union(out, p, u);
-    if (CONN == 8) {
-      if (gx > 0 && !u_same && !left_same && labels[u - 1] == l)
-        g_union(out, p, u - 1);
-      if (gx + 1 < W && !u_same && labels[u + 1] == l) g_union(out, p, u + 1);
-    }
-  } else if (side == 1) {
-    if (gx == 0) return;
-    const bool up_in = gy > y0;
-    const bool lf_same = labels[p - 1] == l;
-    const bool up_same = up_in && labels[p - W] == l;
-    if (lf_same && !(up_same && labels[p - W - 1] == l)) g_union(out, p, p - 1);
-    if (CONN == 8 && up_in && !lf_same && !up_same && labels[p - W - 1] == l)
-      g_union(out, p, p - W - 1);
-  } else {
-    if (gy == y0 || gx + 1 >= W) return;
-    const int ur = p - W + 1;
-    if (labels[ur] == l && labels[p - W] != l && labels[p + 1] != l)
-      g_union(out, p, ur);
-  }
+  ccl_merge_body<CONN>(labels, W, H, out, blockIdx.x, blockIdx.y, threadIdx.x);
 }
 
 // Every pixel follows its chain to the root.  With `roots` non-null, wave w
@@ -216,18 +53,7 @@ __global__ void ccl_flatten(int32_t* out, int64_t n, uint32_t* roots) {
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t p = base + threadIdx.x;
-    bool root = false;
-    if (p < n) {
-      const int r0 = out[p];
-      int r = r0;
-      for (;;) {
-        const int q = out[r];
-        if (q == r) break;
-        r = q;
-      }
-      if (r != r0) out[p] = r;
-      root = r == p;
-    }
+    const bool root = p < n && ccl_flatten_one(out, p) == p;
     const uint64_t m = ballot(root);
     if (roots && lane == 0 && p < n) {
       const int64_t w = p >> 5;  // p (lane 0) is a multiple of 64
@@ -480,19 +306,19 @@ int connected_components_impl(const int32_t* labels, int W, int H,
   if (conn != 4 && conn != 8)
     return fail(SPLIT_HIP_ERR_ARGUMENT, "connectivity %d is not 4 or 8",
                 conn);
-  const int ntx = (W + kTile - 1) / kTile, nty = (H + kTile - 1) / kTile;
+  const int ntx = (W + kCcTile - 1) / kCcTile, nty = (H + kCcTile - 1) / kCcTile;
   if (nty > 65535)
     return fail(SPLIT_HIP_ERR_UNSUPPORTED, "height %d too large", H);
   const dim3 tiles(ntx, nty);
   if (conn == 8) {
-    hipLaunchKernelGGL(ccl_tile<8>, tiles, dim3(kTileThreads), 0, stream,
+    hipLaunchKernelGGL(ccl_tile<8>, tiles, dim3(kCcThreads), 0, stream,
                        labels, W, H, out);
-    hipLaunchKernelGGL(ccl_merge<8>, tiles, dim3(3 * kTile), 0, stream,
+    hipLaunchKernelGGL(ccl_merge<8>, tiles, dim3(3 * kCcTile), 0, stream,
                        labels, W, H, out);
   } else {
-    hipLaunchKernelGGL(ccl_tile<4>, tiles, dim3(kTileThreads), 0, stream,
+    hipLaunchKernelGGL(ccl_tile<4>, tiles, dim3(kCcThreads), 0, stream,
                        labels, W, H, out);
-    hipLaunchKernelGGL(ccl_merge<4>, tiles, dim3(3 * kTile), 0, stream,
+    hipLaunchKernelGGL(ccl_merge<4>, tiles, dim3(3 * kCcTile), 0, stream,
                        labels, W, H, out);
   }
   SPLIT_HIP_CHECK_LAUNCH("ccl_tile/ccl_merge");

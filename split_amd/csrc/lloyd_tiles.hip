@@ -31,11 +31,11 @@
 
 #include "common.hpp"
 #include "lloyd_device.hpp"
+#include "lloyd_sums.hpp"
 
 namespace splithip {
 
 constexpr int kDmSlots = 256;    // drift history entries; t0 is 8 bits
-constexpr int kRunFlushTiles = 64;
 
 struct TileArgs : Geometry {
   int k;
@@ -47,93 +47,6 @@ struct TileArgs : Geometry {
   LloydState* state;
   Mailbox* mbox;
 };
-
-__device__ __forceinline__ int wave_sum_i32(int x) {
-  return (int)wave_sum_u32((unsigned)x);
-}
-
-// Exact wave sum of int32 values whose total may exceed 32 bits.
-__device__ __forceinline__ long long wave_sum_wide(int s) {
-  const int lo = s & 0xFFFF;  // 0..65535
-  const int hi = s >> 16;     // arithmetic shift: s = hi * 65536 + lo
-  return ((long long)wave_sum_i32(hi) << 16) + (long long)wave_sum_i32(lo);
-}
-
-__device__ __forceinline__ void slice_add64(long long* slice, int k, int label,
-                                            long long sx, long long sy,
-                                            long long sz, long long n,
-                                            int lane) {
-  if (lane == 0) {
-    slice[label] += sx;
-    slice[k + label] += sy;
-    slice[2 * k + label] += sz;
-    slice[3 * k + label] += n;
-  }
-}
-
-// Register run of a wave-uniform label: per-lane int32 sums of quantised
-// pixels, flushed (exact wide wave sums) on a label change or every
-// kRunFlushTiles tiles, which bounds a lane's sum by 2^30.
-struct IRun {
-  int x, y, z, n;
-  int label;
-  int tiles;
-};
-
-__device__ __forceinline__ void irun_flush(IRun& r, long long* slice, int k,
-                                           int lane) {
-  if (r.label >= 0) {
-    const long long sx = wave_sum_wide(r.x);
-    const long long sy = wave_sum_wide(r.y);
-    const long long sz = wave_sum_wide(r.z);
-    const long long sn = wave_sum_i32(r.n);
-    slice_add64(slice, k, r.label, sx, sy, sz, sn, lane);
-  }
-  r.x = r.y = r.z = r.n = 0;
-  r.label = -1;
-  r.tiles = 0;
-}
-
-// Adds sign * (q, 1) of each pixel with lab[i] >= 0 into its label's slice
-// entries, one distinct label at a time (|per-lane sum| <= 2^24).
-__device__ __forceinline__ void add_by_label(long long* slice, int k, int lane,
-                                             const int (&lab)[4],
-                                             const int (&qx)[4],
-                                             const int (&qy)[4],
-                                             const int (&qz)[4], int sign) {
-  unsigned rem = 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) rem |= (lab[i] >= 0 ? 1u : 0u) << i;
-  for (;;) {
-    const uint64_t act = ballot(rem != 0);
-    if (act == 0) break;
-    const int first_lane = __builtin_ctzll(act);
-    const int mine = (rem & 1u) ? lab[0]
-                     : (rem & 2u) ? lab[1]
-                     : (rem & 4u) ? lab[2]
-                     : (rem & 8u) ? lab[3] : -1;
-    const int L = readlane_i(mine, first_lane);
-    int sx = 0, sy = 0, sz = 0, cnt = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const bool m = ((rem >> i) & 1u) && lab[i] == L;
-      sx += m ? qx[i] : 0;
-      sy += m ? qy[i] : 0;
-      sz += m ? qz[i] : 0;
-      cnt += m ? 1 : 0;
-      rem &= m ? ~(1u << i) : ~0u;
-    }
-    const long long tx = wave_sum_i32(sx), ty = wave_sum_i32(sy),
-                    tz = wave_sum_i32(sz), tn = wave_sum_i32(cnt);
-    slice_add64(slice, k, L, sign * tx, sign * ty, sign * tz, sign * tn, lane);
-  }
-}
-
-__device__ __forceinline__ float scale_of(unsigned maxabs_bits) {
-  int e = 0;
-  frexpf(__uint_as_float(maxabs_bits), &e);  // max|v| <= 2^e (0 -> e = 0)
-  return ldexpf(1.f, 22 - e);
-}
 
 // fp16 bits of s >= 0 rounded down (to the next smaller half when the
 // nearest one is above s; a value past the fp16 range becomes 65504).

@@ -15,6 +15,10 @@ Workloads (name, size, clustering, connected components):
   stress_8192 8192x8192, 20000 sites + uniform noise in [-0.05, 0.05],
               k=256; plus the 8192x8192 spiral/checkerboard label image for
               connected components only, 8-connectivity
+  batch_256   256 images generated as shaded_4096 (shading, sigma 0.01),
+              each side drawn from 128..2048, 50..2000 sites, k drawn from
+              {4, 8, 16, 32, 64}; stop at shift < 1e-4 or 100 iterations,
+              8-connectivity (see batch_workloads)
 """
 from __future__ import annotations
 
@@ -137,3 +141,27 @@ def scaled(w: Workload, width: int, height: int) -> Workload:
                    image=replace(w.image, width=width, height=height, sites=sites,
                                  base_cell=w.image.base_cell or max(w.image.width,
                                                                     w.image.height) / 8.0))
+
+
+BATCH_KS = (4, 8, 16, 32, 64)
+
+
+def batch_workloads(count: int = 256, seed: int = 0x5A17_0003,
+                    side_range: tuple[int, int] = (128, 2048)) -> list[Workload]:
+    """The images of a batched run: sizes, site counts, k and seeds drawn
+    from a seeded PCG64 stream, each image generated as shaded_4096."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo, hi = side_range
+    out = []
+    for i in range(count):
+        w = int(rng.integers(lo, hi + 1))
+        h = int(rng.integers(lo, hi + 1))
+        sites = int(rng.integers(50, 2001))
+        k = int(BATCH_KS[int(rng.integers(0, len(BATCH_KS)))])
+        img_seed = int(rng.integers(0, 2**63))
+        init_seed = int(rng.integers(0, 2**63))
+        out.append(Workload(
+            f"batch[{i}]",
+            ImageSpec(w, h, seed=img_seed, sites=sites, shading=True, noise_sigma=0.01),
+            k=k, init_seed=init_seed, max_iter=100, threshold=1e-4, connectivity=8))
+    return out

@@ -43,6 +43,18 @@ def test_host_only_entry_points_without_gpu():
     n = 4096 * 4096
     assert 4 * n <= ws < 4 * n + (1 << 20)  # per-pixel Lloyd state + small tables
     assert lib.split_hip_compress_labels_workspace(1 << 20, 1 << 20) < 4 << 20
+    # batch workspace: sizes only (buffers may be null), k limited to 64
+    descs = (sa.ImageDesc * 3)()
+    for i, (w, h, k) in enumerate([(2048, 2048, 64), (37, 53, 4), (1, 1, 1)]):
+        descs[i] = sa.ImageDesc(None, None, None, None, w, h, k, 0)
+    ws = lib.split_hip_segment_batch_workspace(descs, 3)
+    n = 2048 * 2048 + 37 * 53 + 1
+    assert n // 8 <= ws < n // 4 + (1 << 16)
+    descs[1].k = 65
+    assert lib.split_hip_segment_batch_workspace(descs, 3) == 0
+    descs[1].k = 4
+    descs[2].width = 0
+    assert lib.split_hip_segment_batch_workspace(descs, 3) == 0
 
 
 def test_argument_errors_need_no_gpu():
