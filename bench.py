@@ -6,12 +6,19 @@ HBM (split_hip_uniform_random_initialize + split_hip_segment).  Default
 workload: shaded_4096 (4096x4096 Voronoi x Lambertian shading, k=32, stop at
 centroid shift < 1e-4 or 100 iterations, 8-connectivity).
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the fused Lloyd
-step): 12 bytes/pixel algorithmic traffic (three float planes) per launch,
-over its average launch time measured with HIP events on the stream it runs
-on, as the slope of cluster() time between two fixed iteration counts.
+Prints one JSON line.  `roofline` is for the dominant kernel (the bounded
+Lloyd step, tiles_step): 12 bytes/pixel algorithmic traffic (three float
+planes) per launch, over its average launch time measured with HIP events on
+the stream it runs on, as the slope of cluster() time between two fixed
+iteration counts (8 and 40).  `traffic` is the HBM bytes per launch of the
+same launches, from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) of
+`bench.py --profile-driver`, scaled by the counters of a 256 MiB copy.
 `cpu_baseline` times the oracle's restatement of the reference pipeline on a
 bounded crop of the same image, single-threaded, on this host.
+
+`--workload batch_256` instead runs the batched path (256 images of uneven
+size and k, one graph-replayed launch sequence) against the same images
+through 256 separate segment calls.
 """
 from __future__ import annotations
 
@@ -37,18 +44,27 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--cpu-crop", type=int, default=1024)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes behind roofline.traffic")
     ap.add_argument("--profile-driver", action="store_true",
-                    help="run the fixed workload the counter profiles use and exit: "
-                         "a 256 MiB device copy (traffic scale reference), then 20 "
-                         "tiled Lloyd iterations, 8-connectivity CC and compress")
+                    help="run the fixed workload the counter passes profile and exit: "
+                         "a 256 MiB device copy (traffic scale reference), the roofline "
+                         "leg's 40-iteration tiled cluster(), CC and compress")
     return ap.parse_args()
 
 
+ROOFLINE_LO, ROOFLINE_HI = 8, 40   # fixed iteration counts of the slope timing
+COPY_BYTES = 256 << 20             # traffic scale reference
+
+
 def profile_driver(w):
+    """The fixed run the counter passes profile: a 256 MiB device copy (the
+    scale reference), then exactly the roofline leg's longer cluster() run
+    (seeded init, ROOFLINE_HI iterations, tiled path), then CC + compress."""
     import torch
     import split_amd as sa
     from split_amd import workloads as W
-    ref = torch.empty(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+    ref = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
     ref.fill_(1.0)
     ref.clone()
     torch.cuda.synchronize()
@@ -58,11 +74,54 @@ def profile_driver(w):
     L = torch.empty(w.n, dtype=torch.int32, device="cuda")
     S = torch.empty(w.n, dtype=torch.int32, device="cuda")
     sa.uniform_random_initialize(P, C, w.init_seed)
-    it, _ = sa.cluster(P, C, L, 20, -1.0, width=width, height=height)
+    it, _ = sa.cluster(P, C, L, ROOFLINE_HI, -1.0, width=width, height=height)
     sa.connected_components(L.view(height, width), S, connectivity=w.connectivity)
     n = sa.compress_labels(S)
     torch.cuda.synchronize()
     log(f"profile driver: {it} iterations, {n} segments")
+
+
+def _counter_rows(exe, counter, workload, outdir):
+    import csv
+    import subprocess
+    name = counter.lower()
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", str(outdir), "-o", name,
+           "--", sys.executable, str(ROOT / "bench.py"), "--workload", workload,
+           "--profile-driver"]
+    env = dict(os.environ, TMPDIR="/tmp")
+    subprocess.run(cmd, capture_output=True, text=True, timeout=180, env=env, check=True)
+    files = sorted(Path(outdir).rglob(f"{name}_counter_collection.csv"))
+    if not files:
+        raise RuntimeError(f"no {name} counter file")
+    with open(files[0], newline="") as f:
+        return [(r["Kernel_Name"], float(r["Counter_Value"])) for r in csv.DictReader(f)]
+
+
+def measure_traffic(workload):
+    """HBM bytes per bounded Lloyd launch over the roofline leg's iterations,
+    from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) of the profile
+    driver, each scaled by what the counter reads for the 256 MiB copy."""
+    import shutil
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    res = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            rows = _counter_rows(exe, counter, workload, Path(d) / counter.lower())
+            copy = [v for k, v in rows if "copyBuffer" in k]
+            steps = [v for k, v in rows if k.startswith("void splithip::tiles_step")
+                     and "false>" in k]
+            want = ROOFLINE_HI - 1
+            if not copy or len(steps) != want:
+                raise RuntimeError(f"{counter}: {len(copy)} copies, {len(steps)} bounded steps "
+                                   f"(want {want})")
+            scale = COPY_BYTES / (max(copy) * 1024.0)
+            sel = steps[ROOFLINE_LO - 1:]          # iterations LO+1 .. HI
+            res[counter] = (sum(sel) / len(sel) * 1024.0 * scale, scale)
+    (rd, rs), (wr, ws) = res["FETCH_SIZE"], res["WRITE_SIZE"]
+    return {"read": rd, "write": wr, "read_scale": rs, "write_scale": ws}
 
 
 def batch_bench(args):
@@ -295,14 +354,14 @@ def main():
             return a.elapsed_time(b)
 
         timed(4)
-        lo, hi = 8, 40
+        lo, hi = ROOFLINE_LO, ROOFLINE_HI
         t_lo = min(timed(lo) for _ in range(3))
         t_hi = min(timed(hi) for _ in range(3))
         per_launch_ms = (t_hi - t_lo) / (hi - lo)
         bytes_per_launch = 12 * n
         achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
         out["roofline"] = {
-            "kernel": "lloyd_step (assign + accumulate + finalize)",
+            "kernel": "tiles_step<1, 3, false> (bounded Lloyd iteration: skip test, re-assign, exact sums, finalize)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -313,6 +372,21 @@ def main():
             "bytes_per_launch": bytes_per_launch,
         }
         log(f"bench: lloyd_step {per_launch_ms * 1e3:.2f} us/launch -> {achieved:.1f} GB/s")
+        if not args.no_traffic:
+            log("bench: counter passes for roofline.traffic")
+            try:
+                t = measure_traffic(w.name)
+            except Exception as e:  # the profiler is optional; say why it is absent
+                t = None
+                out["roofline"]["traffic_error"] = str(e)[:200]
+            if t is not None:
+                out["roofline"]["traffic"] = round(t["read"] + t["write"])
+                out["roofline"]["traffic_read"] = round(t["read"])
+                out["roofline"]["traffic_write"] = round(t["write"])
+                out["roofline"]["traffic_scale"] = [round(t["read_scale"], 4),
+                                                    round(t["write_scale"], 4)]
+                log(f"bench: traffic {t['read'] / 1e6:.1f} MB read + {t['write'] / 1e6:.1f} MB "
+                    f"written per launch")
 
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
