@@ -20,6 +20,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "split_gen.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -235,6 +237,97 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
                             int32_t compress, int32_t use_graph,
                             split_hip_batch_result* results,
                             split_hip_stream_t stream);
+
+/* ---- seeded synthetic images on the device ------------------------------
+ * No counterpart in the reference (whose inputs are image files loaded by
+ * samples/src/main.cu:81-132).  The construction of split_gen_image
+ * (split_gen.h) run on the GPU for many images at once: same hashes and
+ * geometry; noisy pixels may differ from the host image by one float ulp
+ * (device log/cos). */
+typedef struct split_hip_gen_desc {
+  float* planes;  /* out: 3 SoA planes of width*height floats */
+  int32_t width, height;
+  split_gen_params params;
+} split_hip_gen_desc;
+
+size_t split_hip_gen_images_workspace(const split_hip_gen_desc* images,
+                                      int32_t count);
+/* Returns after the images are written. */
+int split_hip_gen_images(const split_hip_gen_desc* images, int32_t count,
+                         void* temp, size_t temp_bytes,
+                         split_hip_stream_t stream);
+
+/* ---- learned warm start --------------------------------------------------
+ * No counterpart in the reference: its cluster() takes the initial
+ * centroids as an in/out argument (split/include/split/device/kmeans/
+ * cluster.cuh:19-22,31-37) and its caller fills them by random sampling
+ * (samples/src/main.cu:230-233).  This network predicts those 16 starting
+ * centroids from the image.
+ *
+ * Network: input (B, 256, 256, 3) bf16 NHWC; four 3x3 stride-2 convolutions
+ * 3->16->32->64->64 with ReLU; global average pool; dense 64->128 ReLU;
+ * dense 128->48 with sigmoid = 16 centroids in the (3, k) layout.
+ * Parameters: split_hip_ws_param_count() floats; layer l holds W_l[n][Kp_l]
+ * row-major, columns (ky*3+kx)*Cin + ci (conv) or ci (dense), the bias at
+ * column K_l (= 9*Cin or Cin), zeros up to Kp_l = roundup(K_l + 1, 32);
+ * layers in order conv1..conv4, dense5, dense6.  Weights and activations
+ * are rounded to bf16 for the matrix cores, which accumulate in fp32. */
+#define SPLIT_HIP_WS_K 16
+#define SPLIT_HIP_WS_SIDE 256
+
+size_t split_hip_ws_param_count(void);
+/* Bytes of ws for a batch of `batch` images (activations and gradients). */
+size_t split_hip_ws_workspace(int32_t batch);
+/* Resamples images (points, width, height of each desc) to the network
+ * input: `input` = count x 256 x 256 x 3 bf16 (uint16 bits), area average.
+ * temp holds count * 16 bytes.  Returns after the input is written. */
+int split_hip_ws_pack(const split_hip_image_desc* images, int32_t count,
+                      uint16_t* input, void* temp, size_t temp_bytes,
+                      split_hip_stream_t stream);
+/* Forward pass over `batch` inputs: input slot b is image index[b] of
+ * `input` (index NULL: slot b).  out (batch x 48 floats, may be NULL)
+ * receives the sigmoid outputs.  Keeps the activations in ws. */
+int split_hip_ws_forward(const float* params, const uint16_t* input,
+                         const int32_t* index, int32_t batch, void* ws,
+                         size_t ws_bytes, float* out,
+                         split_hip_stream_t stream);
+/* Loss against targets[index[b]] (48 floats per image) and its gradient
+ * w.r.t. every parameter, written to grads (param_count floats).
+ * SPLIT_HIP_WS_LOSS_MSE: (1/B) sum (out - target)^2 slot by slot;
+ * SPLIT_HIP_WS_LOSS_CHAMFER: (1/B) sum over images of the Chamfer set
+ * distance sum_t min_p |t-p|^2 + sum_p min_t |p-t|^2 between the 16
+ * predicted and the 16 target colours (batch <= 256).  Uses the activations
+ * of the preceding split_hip_ws_forward with the same params, input, index,
+ * batch and ws.  loss (device float, may be NULL) receives the loss. */
+#define SPLIT_HIP_WS_LOSS_MSE 0
+#define SPLIT_HIP_WS_LOSS_CHAMFER 1
+int split_hip_ws_backward(const float* params, const uint16_t* input,
+                          const float* targets, const int32_t* index,
+                          int32_t batch, int32_t loss_kind, void* ws,
+                          size_t ws_bytes, float* grads, float* loss,
+                          split_hip_stream_t stream);
+/* One Adam step (bias-corrected, step >= 1) on params with moments m, v. */
+int split_hip_ws_adam(float* params, const float* grads, float* m, float* v,
+                      int32_t step, float lr, float beta1, float beta2,
+                      float eps, split_hip_stream_t stream);
+/* Predicted starting centroids for each image: resample, forward, and write
+ * the 16 centroids to images[i].centroids (k must be 16).  ws holds
+ * split_hip_ws_workspace(count) bytes.  Returns after they are written. */
+int split_hip_ws_predict(const float* params,
+                         const split_hip_image_desc* images, int32_t count,
+                         void* ws, size_t ws_bytes, split_hip_stream_t stream);
+/* Training targets: per image, its 16 centroids (3 x 16 floats) reordered
+ * by luminance 0.299 R + 0.587 G + 0.114 B, ties to the lower index. */
+int split_hip_ws_targets(const float* centroids, int32_t count, float* targets,
+                         split_hip_stream_t stream);
+
+/* k-means inertia sum_i min_j |p_i - c_j|^2 (float distances in the
+ * assignment's form, summed in double in a fixed order).  No counterpart in
+ * the reference, which reports no inertia.  Host-synchronous. */
+size_t split_hip_inertia_workspace(void);
+int split_hip_inertia(const float* points, int64_t n, const float* centroids,
+                      int32_t k, void* temp, size_t temp_bytes,
+                      double* inertia, split_hip_stream_t stream);
 
 #ifdef __cplusplus
 }

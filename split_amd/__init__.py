@@ -56,6 +56,26 @@ class ImageDesc(ctypes.Structure):
                 ("k", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class GenParams(ctypes.Structure):
+    """split_gen_params (include/split_gen.h)."""
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("sites", ctypes.c_int32),
+        ("shading", ctypes.c_int32),
+        ("octaves", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("base_cell", ctypes.c_double),
+        ("height_amp", ctypes.c_double),
+        ("noise_sigma", ctypes.c_double),
+        ("noise_uniform", ctypes.c_double),
+    ]
+
+
+class GenDesc(ctypes.Structure):
+    _fields_ = [("planes", ctypes.c_void_p), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("params", GenParams)]
+
+
 class BatchResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("shift", ctypes.c_float),
                 ("num_segments", ctypes.c_int32), ("reserved", ctypes.c_int32)]
@@ -109,6 +129,27 @@ CABI = {
                                                _c_size_t, _c_i32, _c_float, _c_i32, _c_i32,
                                                _c_i32, ctypes.POINTER(BatchResult),
                                                _c_void_p]),
+    "split_hip_ws_param_count": (_c_size_t, []),
+    "split_hip_ws_workspace": (_c_size_t, [_c_i32]),
+    "split_hip_ws_pack": (ctypes.c_int, [ctypes.POINTER(ImageDesc), _c_i32, _c_void_p, _c_void_p,
+                                         _c_size_t, _c_void_p]),
+    "split_hip_ws_forward": (ctypes.c_int, [_c_void_p, _c_void_p, _c_void_p, _c_i32, _c_void_p,
+                                            _c_size_t, _c_void_p, _c_void_p]),
+    "split_hip_ws_backward": (ctypes.c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_i32,
+                                             _c_i32, _c_void_p, _c_size_t, _c_void_p,
+                                             _c_void_p, _c_void_p]),
+    "split_hip_ws_adam": (ctypes.c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_i32,
+                                         _c_float, _c_float, _c_float, _c_float, _c_void_p]),
+    "split_hip_ws_predict": (ctypes.c_int, [_c_void_p, ctypes.POINTER(ImageDesc), _c_i32,
+                                            _c_void_p, _c_size_t, _c_void_p]),
+    "split_hip_ws_targets": (ctypes.c_int, [_c_void_p, _c_i32, _c_void_p, _c_void_p]),
+    "split_hip_inertia_workspace": (_c_size_t, []),
+    "split_hip_inertia": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p, _c_i32, _c_void_p,
+                                         _c_size_t, ctypes.POINTER(ctypes.c_double),
+                                         _c_void_p]),
+    "split_hip_gen_images_workspace": (_c_size_t, [ctypes.POINTER(GenDesc), _c_i32]),
+    "split_hip_gen_images": (ctypes.c_int, [ctypes.POINTER(GenDesc), _c_i32, _c_void_p,
+                                            _c_size_t, _c_void_p]),
 }
 
 _lib = None
@@ -377,6 +418,36 @@ def segment_batch(images, *, max_iter: int = 100, threshold: float = 1e-4,
                                      _stream()), "segment_batch")
     return [SegmentResult(int(r.iterations), float(r.shift), int(r.num_segments))
             for r in res]
+
+
+def gen_images(specs, outs, temp=None) -> None:
+    """Seeded synthetic images on the device: outs[i] ((3, w*h) float32 CUDA
+    tensor) <- the image of specs[i] (a workloads.ImageSpec)."""
+    count = len(specs)
+    descs = (GenDesc * count)()
+    for i, (sp, o) in enumerate(zip(specs, outs)):
+        p, n = _points(o)
+        if n != sp.width * sp.height:
+            raise ValueError(f"image {i}: output must hold width*height pixels")
+        descs[i] = GenDesc(p, sp.width, sp.height, sp.params())
+    L = lib()
+    if temp is None:
+        temp = _temp(L.split_hip_gen_images_workspace(descs, count), outs[0])
+    _check(L.split_hip_gen_images(descs, count, temp.data_ptr(), temp.numel(), _stream()),
+           "gen_images")
+
+
+def inertia(points, centroids, temp=None) -> float:
+    """sum_i min_j |p_i - c_j|^2 (float distances, double sum)."""
+    p, n = _points(points)
+    c, k = _cent(centroids)
+    L = lib()
+    if temp is None:
+        temp = _temp(L.split_hip_inertia_workspace(), points)
+    out = ctypes.c_double(0.0)
+    _check(L.split_hip_inertia(p, n, c, k, temp.data_ptr(), temp.numel(), ctypes.byref(out),
+                               _stream()), "inertia")
+    return float(out.value)
 
 
 def version() -> str:

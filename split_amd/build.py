@@ -27,7 +27,7 @@ OBJ = LIB / "obj"
 INCLUDE = ROOT / "include"
 
 ARCH = "gfx950"
-HIP_SOURCES = ["kmeans.hip", "lloyd_tiles.hip", "ccl.hip", "batch.hip", "capi.hip"]
+HIP_SOURCES = ["kmeans.hip", "lloyd_tiles.hip", "ccl.hip", "batch.hip", "gen_device.hip", "warmstart.hip", "capi.hip"]
 CXX_SOURCES = ["split_api.cpp"]
 GEN_SOURCES = ["gen.c"]
 
@@ -86,7 +86,7 @@ def build_gen() -> Path:
     out = LIB / "libsplit_gen.so"
     tmp = LIB / "libsplit_gen.so.tmp"
     _run(["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
-          "-std=c11", "-Wall", "-Wextra", f"-I{CSRC}",
+          "-std=c11", "-Wall", "-Wextra", f"-I{CSRC}", f"-I{INCLUDE}",
           *[str(CSRC / s) for s in GEN_SOURCES], "-o", str(tmp), "-lm"])
     os.replace(tmp, out)
     return out

@@ -28,21 +28,7 @@ from pathlib import Path
 
 import numpy as np
 
-from . import GEN_LIB, SplitLibraryError
-
-
-class GenParams(ctypes.Structure):
-    _fields_ = [
-        ("seed", ctypes.c_uint64),
-        ("sites", ctypes.c_int32),
-        ("shading", ctypes.c_int32),
-        ("octaves", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
-        ("base_cell", ctypes.c_double),
-        ("height_amp", ctypes.c_double),
-        ("noise_sigma", ctypes.c_double),
-        ("noise_uniform", ctypes.c_double),
-    ]
+from . import GEN_LIB, GenParams, SplitLibraryError
 
 
 _gen = None

@@ -1,0 +1,196 @@
+"""Warm-start network kernels (split_hip_ws_*) against the CPU oracle.
+
+Bars: the packed input is bit-identical to the oracle's resample (rounded to
+bf16); forward outputs and every layer's parameter gradient agree with the
+oracle run with the same bf16 rounding points within 2% relative L2 error
+(fp32 accumulation order and rare one-ulp bf16 rounding flips are the only
+differences); Adam, the target sort and inertia match their closed forms.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import warmstart as ows
+import split_amd as sa
+from split_amd import warmstart as sw
+from split_amd import workloads as W
+from tests.conftest import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits_to_f64(t):
+    a = t.cpu().numpy().astype(np.uint16).astype(np.uint32) << 16
+    return a.view(np.float32).astype(np.float64)
+
+
+def test_param_layout_matches_oracle(torch_cuda):
+    assert sw.param_count() == ows.PARAM_COUNT == int(sa.lib().split_hip_ws_param_count())
+    assert np.array_equal(sw.init_params(3), ows.init_params(3))
+
+
+def test_pack_matches_oracle(torch_cuda):
+    torch = torch_cuda
+    say("ws pack vs oracle resample")
+    specs = [W.ImageSpec(256, 256, seed=1, sites=30, shading=True, noise_sigma=0.01),
+             W.ImageSpec(517, 300, seed=2, sites=40, shading=True),
+             W.ImageSpec(200, 130, seed=3, sites=10)]
+    imgs = []
+    for s in specs:
+        P = torch.from_numpy(W.make_image(s)).cuda()
+        imgs.append(dict(points=P, width=s.width, height=s.height))
+    got = sw.pack(imgs)
+    torch.cuda.synchronize()
+    for i, s in enumerate(specs):
+        want = ows.bf16(ows.pack_input(W.make_image(s), s.width, s.height))
+        assert np.array_equal(_bits_to_f64(got[i]), want), s
+
+
+@pytest.fixture(scope="module")
+def small_batch(torch_cuda):
+    torch = torch_cuda
+    specs = [W.ImageSpec(256, 256, seed=10 + i, sites=25 + 40 * i, shading=True,
+                         noise_sigma=0.01) for i in range(3)]
+    imgs = [dict(points=torch.from_numpy(W.make_image(s)).cuda(), width=256, height=256)
+            for s in specs]
+    X = sw.pack(imgs)
+    torch.cuda.synchronize()
+    Xo = _bits_to_f64(X).reshape(3, 256, 256, 3)
+    rng = np.random.default_rng(5)
+    params = sw.init_params(77)
+    # non-zero biases so every bias path is exercised
+    off = 0
+    for cin, cout, kk in sw._LAYERS:
+        kp = sw._kp(kk)
+        blk = params[off:off + cout * kp].reshape(cout, kp)
+        blk[:, kk] = 0.05 * rng.standard_normal(cout)
+        off += cout * kp
+    T = rng.random((3, 48)).astype(np.float32)
+    return X, Xo, params, T
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("kind", ["mse", "chamfer"])
+def test_forward_and_backward_match_oracle(torch_cuda, small_batch, kind):
+    torch = torch_cuda
+    X, Xo, params, T = small_batch
+    say(f"ws forward/backward vs oracle (B=3, {kind})")
+    net = sw.WarmStartNet(params)
+    out = net.forward(X)
+    grads = torch.zeros(sw.param_count(), dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    Tt = torch.from_numpy(T).cuda()
+    net.backward(X, Tt, grads, loss, kind=kind)
+    torch.cuda.synchronize()
+    acts = ows.forward(params, Xo, mimic=True)
+    g_want, L_want = ows.backward(params, acts, T.astype(np.float64), mimic=True, kind=kind)
+    P = out.cpu().numpy().astype(np.float64)
+    assert np.abs(P - acts["P"]).max() < 2e-3
+    assert abs(float(loss.item()) - L_want) <= 1e-3 * L_want
+    g = grads.cpu().numpy().astype(np.float64)
+    for l, (kind, cin, cout, K, Kp, off) in enumerate(ows.DIMS):
+        gl = g[off:off + cout * Kp].reshape(cout, Kp)
+        wl = g_want[off:off + cout * Kp].reshape(cout, Kp)
+        assert _rel(gl[:, :K + 1], wl[:, :K + 1]) < 2e-2, (l, _rel(gl, wl))
+        assert np.all(gl[:, K + 1:] == 0.0), l
+
+
+def test_index_gather_equals_contiguous(torch_cuda, small_batch):
+    torch = torch_cuda
+    X, Xo, params, T = small_batch
+    net = sw.WarmStartNet(params)
+    a = net.forward(X).clone()
+    idx = torch.tensor([2, 0, 1], dtype=torch.int32, device="cuda")
+    b = net.forward(X, index=idx, batch=3)
+    torch.cuda.synchronize()
+    assert torch.equal(b, a[[2, 0, 1]])
+
+
+def test_adam_step(torch_cuda):
+    torch = torch_cuda
+    n = sw.param_count()
+    rng = np.random.default_rng(8)
+    p0 = rng.standard_normal(n).astype(np.float32)
+    g = rng.standard_normal(n).astype(np.float32)
+    m0 = (0.1 * rng.standard_normal(n)).astype(np.float32)
+    v0 = np.abs(rng.standard_normal(n)).astype(np.float32)
+    P, G, M, V = (torch.from_numpy(x.copy()).cuda() for x in (p0, g, m0, v0))
+    sw.adam(P, G, M, V, step=3, lr=1e-2)
+    torch.cuda.synchronize()
+    m = 0.9 * m0 + 0.1 * g
+    v = 0.999 * v0 + 0.001 * g * g
+    want = p0 - 1e-2 * (m / (1 - 0.9 ** 3)) / (np.sqrt(v / (1 - 0.999 ** 3)) + 1e-8)
+    assert np.allclose(P.cpu().numpy(), want, rtol=1e-5, atol=1e-6)
+
+
+def test_targets_sorted_by_luminance(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(9)
+    C = rng.random((5, 3, 16)).astype(np.float32)
+    C[0, :, 3] = C[0, :, 7]  # a tie: lower index first
+    got = sw.targets(torch.from_numpy(C).cuda(), 5).cpu().numpy()
+    for i in range(5):
+        assert np.array_equal(got[i].reshape(3, 16), ows.sort_by_luminance(C[i]))
+
+
+def test_inertia_matches_oracle(torch_cuda):
+    torch = torch_cuda
+    img = W.make_image(W.ImageSpec(300, 211, seed=4, sites=30, noise_sigma=0.02))
+    C = img[:, [5, 900, 20000, 41000, 7]].copy()
+    got = sa.inertia(torch.from_numpy(img).cuda(), torch.from_numpy(np.ascontiguousarray(C)).cuda())
+    C = np.ascontiguousarray(C)
+    want = oracle.inertia(img, C, oracle.ref_label_points(C, img))
+    assert abs(got - want) <= 1e-6 * want
+
+
+def test_training_steps_reduce_loss(torch_cuda, small_batch):
+    torch = torch_cuda
+    X, Xo, params, T = small_batch
+    say("ws: 30 Adam steps on a fixed batch")
+    net = sw.WarmStartNet(params)
+    n = sw.param_count()
+    grads = torch.zeros(n, dtype=torch.float32, device="cuda")
+    m = torch.zeros_like(grads)
+    v = torch.zeros_like(grads)
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    Tt = torch.from_numpy(T).cuda()
+    first = None
+    for step in range(1, 31):
+        net.forward(X)
+        net.backward(X, Tt, grads, loss)
+        if first is None:
+            first = float(loss.item())
+        sw.adam(net.params, grads, m, v, step, 3e-3)
+    net.forward(X)
+    net.backward(X, Tt, grads, loss)
+    assert float(loss.item()) < 0.5 * first
+
+
+def test_trained_network_gain(torch_cuda):
+    """What the committed network gains over seeded-sample init on 64 held-out
+    1024x1024 config-2-style images: fewer Lloyd iterations to shift < 1e-4
+    on average, at a total final inertia no more than 0.5% worse."""
+    say("ws: trained network vs seeded init on 64 eval images")
+    net = sw.WarmStartNet.load()
+    res = sw.evaluate(net, count=64, side=1024, log=None)["summary"]
+    say(f"   plain iters {res['plain_iters_mean']:.2f} warm {res['warm_iters_mean']:.2f} "
+        f"inertia ratio {res['inertia_ratio_total']:.4f}")
+    assert res["warm_iters_mean"] < res["plain_iters_mean"]
+    assert res["inertia_ratio_total"] <= 1.005
+
+
+def test_predict_shapes_and_errors(torch_cuda):
+    torch = torch_cuda
+    net = sw.WarmStartNet.load()
+    P = torch.from_numpy(W.make_image(W.ImageSpec(300, 200, seed=3, sites=20))).cuda()
+    C = torch.full((3, 16), -1.0, dtype=torch.float32, device="cuda")
+    net.predict([dict(points=P, width=300, height=200, centroids=C)])
+    torch.cuda.synchronize()
+    c = C.cpu().numpy()
+    assert np.all((c > 0) & (c < 1))
+    with pytest.raises(ValueError):
+        net.predict([dict(points=P, width=300, height=200,
+                          centroids=torch.zeros((3, 8), device="cuda"))])
