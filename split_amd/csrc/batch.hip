@@ -2,19 +2,28 @@
 // one captured hipGraph of persistent launches.
 //
 // The images' 16x16 Lloyd tiles are numbered consecutively (image i owns
-// global tiles [tile0_i, tile0_{i+1})).  Each launch of batch_step is one
-// Lloyd iteration of every image that has not stopped: a persistent grid
-// whose waves each walk a contiguous run of global tiles, switching image
+// global tiles [tile0_i, tile0_{i+1})).  Images are split into two groups,
+// each walked by its own persistent kernel once per Lloyd iteration:
+//  * plain images (k < 32): every iteration assigns every pixel of every
+//    tile (box-pruned exact assignment) and accumulates; nothing is stored
+//    per pixel, the labels come from one assignment pass at the end;
+//  * bounded images (k >= 32): the single-image bounded iteration
+//    (lloyd_bounded.hpp) — the first iteration records each pixel's label
+//    and slack, later ones skip the pixels whose bound proves the label
+//    unchanged and re-assign the rest; the labels come from that state.
+// A wave walks a contiguous run of its group's tiles, switching image
 // (centroids, k, scale) where the run crosses an image boundary.  Sums are
 // exact integers (quantised as in the single-image tiled path), so every
-// image gets exactly the centroids and labels it gets on its own.  The wave
-// that completes an image's last tile of the iteration (per-image tile
-// counter) finalises that image: new centroids, shift, stop flag.  Images
-// finish independently; later launches skip their tiles.
+// image gets exactly the centroids, labels and stopping iteration it gets on
+// its own.  The wave that completes an image's last tile of the iteration
+// (per-image tile counter) finalises that image: new centroids, shift, drift
+// history, stop flag.  Images finish independently; later launches skip
+// their tiles.
 //
-// After the iterations: the final label pass, connected components
-// (32x32 tiles, all images in one grid), the root bitmap, a per-image scan
-// and the rank map — the same passes as the single-image path.
+// After the iterations: the labels (from the per-pixel state), connected
+// components (32x32 tiles, all images in one grid), the root bitmap, a
+// per-image scan and the rank map — the same passes as the single-image
+// path.
 //
 // The whole sequence is captured once into a hipGraph per (temp, batch
 // shape, settings) and replayed; per-image buffer pointers live in a
@@ -32,6 +41,7 @@
 #include "common.hpp"
 #include "lloyd_device.hpp"
 #include "lloyd_sums.hpp"
+#include "lloyd_bounded.hpp"
 
 namespace splithip {
 
@@ -53,11 +63,14 @@ struct BatchImg {
   long long* totals;     // 4k
   unsigned* bitmap;      // 2*ceil(n/64) words
   unsigned* prefix;      // ceil(n/32) words
+  uint32_t* pstate;      // per-pixel Lloyd state (lloyd_bounded.hpp), n
+  float* dm;             // cumulative max drift, kDmSlots
   // device state
   int cur, iter, done, nseg;
   float shift;
   unsigned maxabs;
-  int tiles_done, pad1;
+  int tiles_done;
+  int bounded;  // iterations after the first are bounded (keep the totals)
 };
 
 struct BatchArgs {
@@ -69,6 +82,20 @@ struct BatchArgs {
   int max_iter;
   float threshold;
 };
+
+// The images of one Lloyd kernel: img[0..count) of the table, whose tiles
+// are the global tiles [t0, t1).
+struct Group {
+  BatchImg* img;
+  int count;
+  int64_t t0, t1;
+};
+
+// Bounded iterations pay for reading and writing the per-pixel state and
+// for queueing; they win once the full assignment is expensive (k >= 32 on
+// the shaded workloads) and need the drift history of every set (t0 is 8
+// bits).
+constexpr int kBoundedMinK = 32;
 
 // Image owning global index g of the field at byte offset OFF (tile0,
 // ctile0 or px0): the last image whose start is <= g.
@@ -110,6 +137,7 @@ __global__ void batch_init(BatchArgs a) {
     m.shift = 0.f;
     m.maxabs = 0u;
     m.tiles_done = 0;
+    m.dm[0] = 0.f;
   }
 }
 
@@ -137,18 +165,26 @@ __global__ __launch_bounds__(256) void batch_absmax(BatchArgs a) {
   }
 }
 
-// Per-wave accumulation slice [4][kBatchMaxK] of int64 and 64 doubles of
-// finalisation scratch live in LDS.
+// Per-wave LDS: accumulation slice [4][kBatchMaxK] of int64 and 64 doubles
+// of finalisation scratch; the bounded kernel adds the drift history and the
+// queue of failing pixels.
+struct WaveLdsPlain {
+  long long slice[4 * kBatchMaxK];
+  double scratch[kWave];
+};
 struct WaveLds {
   long long slice[4 * kBatchMaxK];
   double scratch[kWave];
+  float dmt[kDmSlots];
+  int qidx[kQueueCap];
+  int qold[kQueueCap];
 };
 
 // Flushes the wave's slice for image m into its totals, then counts the
 // wave's tiles; the wave that completes the image's tiles finalises it.
-__device__ void batch_finish_segment(const BatchArgs& a, int i, int ntiles,
-                                     WaveLds& w, int lane) {
-  BatchImg& m = a.img[i];
+template <class L>
+__device__ void batch_finish_segment(const BatchArgs& a, BatchImg& m,
+                                     int ntiles, L& w, int lane) {
   const int k = m.k;
   for (int v = lane; v < 4 * k; v += kWave) {
     const long long t = w.slice[v];
@@ -189,13 +225,22 @@ __device__ void batch_finish_segment(const BatchArgs& a, int i, int ntiles,
     }
   }
   w.scratch[lane] = d2;
-  for (int v = lane; v < 4 * k; v += kWave) m.totals[v] = 0;
+  // bounded iterations update the totals by label changes; iterations that
+  // re-assign every pixel start from zero
+  if (!m.bounded)
+    for (int v = lane; v < 4 * k; v += kWave) m.totals[v] = 0;
   __threadfence();
   if (lane == 0) {
-    double s = 0.0;
-    for (int j = 0; j < kWave; ++j) s += w.scratch[j];
+    double s = 0.0, mx = 0.0;
+    for (int j = 0; j < kWave; ++j) {
+      s += w.scratch[j];
+      mx = fmax(mx, w.scratch[j]);
+    }
     const float shift = (float)sqrt(s);
     const int it = m.iter + 1;
+    // rounded up: the skip test must never under-estimate the drift
+    if (it < kDmSlots)
+      m.dm[it] = (float)((double)m.dm[it - 1] + sqrt(mx) * (1.0 + 1e-5) + 1e-7);
     m.shift = shift;
     m.iter = it;
     m.cur = 1 - m.cur;
@@ -204,26 +249,28 @@ __device__ void batch_finish_segment(const BatchArgs& a, int i, int ntiles,
   }
 }
 
-// One Lloyd iteration of every active image (see the file comment).
-__global__ __launch_bounds__(1024) void batch_step(BatchArgs a) {
-  extern __shared__ WaveLds wlds[];
+// One Lloyd iteration of the plain images: assign every pixel (exact
+// box-pruned argmin), accumulate quantised sums.
+__global__ __launch_bounds__(1024) void batch_step_plain(BatchArgs a, Group g) {
+  extern __shared__ WaveLdsPlain plds[];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int nwaves = blockDim.x / kWave;
-  WaveLds& w = wlds[wave];
+  WaveLdsPlain& w = plds[wave];
   for (int v = lane; v < 4 * kBatchMaxK; v += kWave) w.slice[v] = 0;
 
   const int64_t gw = (int64_t)blockIdx.x * nwaves + wave;
   const int64_t W = (int64_t)gridDim.x * nwaves;
-  const int64_t s0 = a.tiles * gw / W;
-  const int64_t s1 = a.tiles * (gw + 1) / W;
+  const int64_t n = g.t1 - g.t0;
+  const int64_t s0 = g.t0 + n * gw / W;
+  const int64_t s1 = g.t0 + n * (gw + 1) / W;
 
   int64_t s = s0;
   while (s < s1) {
-    const int i = find_image<&BatchImg::tile0>(a.img, a.count, s);
-    const BatchImg& m = a.img[i];
-    const Geometry g = image_geometry(m);
-    const int64_t end = std::min<int64_t>(s1, m.tile0 + g.nseg);
+    const int i = find_image<&BatchImg::tile0>(g.img, g.count, s);
+    BatchImg& m = g.img[i];
+    const Geometry geo = image_geometry(m);
+    const int64_t end = std::min<int64_t>(s1, m.tile0 + geo.nseg);
     if (m.done) {
       s = end;
       continue;
@@ -239,15 +286,15 @@ __global__ __launch_bounds__(1024) void batch_step(BatchArgs a) {
       cn[0] = fmaf(cz[0], cz[0], fmaf(cy[0], cy[0], cx[0] * cx[0]));
     }
     const float fscale = scale_of(m.maxabs);
-    const bool vec = (g.width % 4 == 0) && ((((uintptr_t)g.pts) & 15u) == 0);
+    const bool vec = (geo.width % 4 == 0) && ((((uintptr_t)geo.pts) & 15u) == 0);
     IRun run{0, 0, 0, 0, -1, 0};
     for (int64_t t = s; t < end; ++t) {
       int64_t idx[4];
       bool v[4], lvec;
-      segment_index<kTileVec>(g, t - m.tile0, lane, idx, v, lvec);
+      segment_index<kTileVec>(geo, t - m.tile0, lane, idx, v, lvec);
       lvec = lvec && vec;
       float x[4], y[4], z[4];
-      load_colours(g, idx, v, lvec, x, y, z);
+      load_colours(geo, idx, v, lvec, x, y, z);
       int lab[4];
       assign_segment<1, false>(cx, cy, cz, cn, k, lane, x, y, z, v, lab,
                                nullptr);
@@ -276,22 +323,88 @@ __global__ __launch_bounds__(1024) void batch_step(BatchArgs a) {
       }
     }
     irun_flush(run, w.slice, k, lane);
-    batch_finish_segment(a, i, (int)(end - s), w, lane);
+    batch_finish_segment(a, m, (int)(end - s), w, lane);
     s = end;
   }
 }
 
-// Labels of the last assignment: every image with at least one iteration
-// is assigned against its previous centroid buffer.
-__global__ __launch_bounds__(256) void batch_labels(BatchArgs a) {
+// One Lloyd iteration of the bounded images.  FIRST: assign every pixel and
+// record its state; otherwise the bounded iteration.
+template <bool FIRST>
+__global__ __launch_bounds__(512) void batch_step_bounded(BatchArgs a, Group g) {
+  extern __shared__ WaveLds wlds[];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int nwaves = blockDim.x / kWave;
+  WaveLds& w = wlds[wave];
+  for (int v = lane; v < 4 * kBatchMaxK; v += kWave) w.slice[v] = 0;
+
+  const int64_t gw = (int64_t)blockIdx.x * nwaves + wave;
+  const int64_t W = (int64_t)gridDim.x * nwaves;
+  const int64_t n = g.t1 - g.t0;
+  const int64_t s0 = g.t0 + n * gw / W;
+  const int64_t s1 = g.t0 + n * (gw + 1) / W;
+
+  int64_t s = s0;
+  while (s < s1) {
+    const int i = find_image<&BatchImg::tile0>(g.img, g.count, s);
+    BatchImg& m = g.img[i];
+    TileArgs ta{};
+    static_cast<Geometry&>(ta) = image_geometry(m);
+    const int64_t end = std::min<int64_t>(s1, m.tile0 + ta.nseg);
+    if (m.done) {
+      s = end;
+      continue;
+    }
+    const int k = m.k;
+    const int it = m.iter;
+    ta.k = k;
+    ta.pstate = m.pstate;
+    const float* cent = m.cbuf + (size_t)m.cur * 3 * k;
+    float cx[1], cy[1], cz[1], cn[1];
+    {
+      const bool live = lane < k;
+      cx[0] = live ? cent[lane] : 0.f;
+      cy[0] = live ? cent[k + lane] : 0.f;
+      cz[0] = live ? cent[2 * k + lane] : 0.f;
+      cn[0] = fmaf(cz[0], cz[0], fmaf(cy[0], cy[0], cx[0] * cx[0]));
+    }
+    const float maxabs = __uint_as_float(m.maxabs);
+    const float fscale = scale_of(m.maxabs);
+    const float margin = 1e-5f * (1.f + 4.f * maxabs);
+    float drift_now = 0.f;
+    if constexpr (!FIRST) {
+      for (int t = lane; t <= it; t += kWave) w.dmt[t] = m.dm[t];
+      __builtin_amdgcn_wave_barrier();
+      drift_now = w.dmt[it];
+    }
+    const bool vec = (ta.width % 4 == 0) && ((((uintptr_t)ta.pts) & 15u) == 0) &&
+                     ((((uintptr_t)ta.pstate) & 15u) == 0);
+    const int64_t t0 = s - m.tile0, t1 = end - m.tile0;
+    if (vec)
+      walk_tiles<1, kTileVec, FIRST>(ta, t0, t1, cx, cy, cz, cn, it, fscale,
+                                     margin, drift_now, w.dmt, w.slice, w.qidx,
+                                     w.qold, lane);
+    else
+      walk_tiles<1, kTile, FIRST>(ta, t0, t1, cx, cy, cz, cn, it, fscale,
+                                  margin, drift_now, w.dmt, w.slice, w.qidx,
+                                  w.qold, lane);
+    batch_finish_segment(a, m, (int)(end - s), w, lane);
+    s = end;
+  }
+}
+
+// Labels of the plain images: the assignment of the last iteration, i.e.
+// against the previous centroid buffer (images that ran an iteration).
+__global__ __launch_bounds__(256) void batch_labels_plain(Group g) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
-  for (int64_t s = gw; s < a.tiles; s += nw) {
-    const int i = find_image<&BatchImg::tile0>(a.img, a.count, s);
-    const BatchImg& m = a.img[i];
+  for (int64_t s = g.t0 + gw; s < g.t1; s += nw) {
+    const int i = find_image<&BatchImg::tile0>(g.img, g.count, s);
+    const BatchImg& m = g.img[i];
     if (m.iter == 0) continue;
-    const Geometry g = image_geometry(m);
+    const Geometry geo = image_geometry(m);
     const int k = m.k;
     const float* cent = m.cbuf + (size_t)(1 - m.cur) * 3 * k;
     float cx[1], cy[1], cz[1], cn[1];
@@ -302,11 +415,11 @@ __global__ __launch_bounds__(256) void batch_labels(BatchArgs a) {
     cn[0] = fmaf(cz[0], cz[0], fmaf(cy[0], cy[0], cx[0] * cx[0]));
     int64_t idx[4];
     bool v[4], lvec;
-    segment_index<kTileVec>(g, s - m.tile0, lane, idx, v, lvec);
-    lvec = lvec && (g.width % 4 == 0) && ((((uintptr_t)g.pts) & 15u) == 0) &&
+    segment_index<kTileVec>(geo, s - m.tile0, lane, idx, v, lvec);
+    lvec = lvec && (geo.width % 4 == 0) && ((((uintptr_t)geo.pts) & 15u) == 0) &&
            ((((uintptr_t)m.labels) & 15u) == 0);
     float x[4], y[4], z[4];
-    load_colours(g, idx, v, lvec, x, y, z);
+    load_colours(geo, idx, v, lvec, x, y, z);
     int lab[4];
     assign_segment<1, false>(cx, cy, cz, cn, k, lane, x, y, z, v, lab, nullptr);
     if (lvec) {
@@ -317,6 +430,26 @@ __global__ __launch_bounds__(256) void batch_labels(BatchArgs a) {
       for (int j = 0; j < 4; ++j)
         if (v[j]) m.labels[idx[j]] = lab[j];
     }
+  }
+}
+
+// Labels of the bounded images, from the per-pixel state (images that ran
+// an iteration).
+__global__ __launch_bounds__(256) void batch_labels_state(Group g) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t s = g.t0 + gw; s < g.t1; s += nw) {
+    const int i = find_image<&BatchImg::tile0>(g.img, g.count, s);
+    const BatchImg& m = g.img[i];
+    if (m.iter == 0) continue;
+    const Geometry geo = image_geometry(m);
+    int64_t idx[4];
+    bool v[4], lvec;
+    segment_index<kTile>(geo, s - m.tile0, lane, idx, v, lvec);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (v[j]) m.labels[idx[j]] = (int32_t)(m.pstate[idx[j]] & 0xFFu);
   }
 }
 
@@ -358,7 +491,8 @@ __global__ __launch_bounds__(256) void batch_flatten(BatchArgs a, int compress) 
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.px;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t p = base + threadIdx.x;
-    const int64_t wave0 = p - lane;
+    const int64_t wave0 = __builtin_amdgcn_readfirstlane(
+        (int)((base + threadIdx.x - lane) >> 6)) * (int64_t)64;
     const int i = find_image<&BatchImg::px0>(a.img, a.count, wave0);
     const BatchImg& m = a.img[i];
     const int64_t n = (int64_t)m.W * m.H;
@@ -404,7 +538,9 @@ __global__ __launch_bounds__(256) void batch_map(BatchArgs a) {
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.px;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t p = base + threadIdx.x;
-    const int i = find_image<&BatchImg::px0>(a.img, a.count, p - lane);
+    const int64_t wave0 = __builtin_amdgcn_readfirstlane(
+        (int)((base + threadIdx.x - lane) >> 6)) * (int64_t)64;
+    const int i = find_image<&BatchImg::px0>(a.img, a.count, wave0);
     const BatchImg& m = a.img[i];
     const int64_t q = p - m.px0;
     if (q < (int64_t)m.W * m.H) {
@@ -419,20 +555,17 @@ __global__ __launch_bounds__(256) void batch_map(BatchArgs a) {
 // ------------------------------------------------------------------ host ---
 
 struct BatchPlan {
-  std::vector<BatchImg> img;
-  size_t table_bytes = 0, total_bytes = 0;
+  std::vector<BatchImg> img;  // table order: plain images, then bounded ones
+  std::vector<int> order;     // table entry -> caller's index
+  int nplain = 0;
+  int64_t tiles_plain = 0;
+  size_t total_bytes = 0;
   int64_t tiles = 0, ctiles = 0, px = 0;
 };
 
 static int make_plan(const split_hip_image_desc* d, int count, void* temp,
-                     BatchPlan* P) {
-  P->img.assign(count, BatchImg{});
-  char* t = static_cast<char*>(temp);
-  size_t o = align_up((size_t)count * sizeof(BatchImg), 256);
-  P->table_bytes = o;
-  int64_t tiles = 0, ctiles = 0, px = 0;
+                     int max_iter, BatchPlan* P) {
   for (int i = 0; i < count; ++i) {
-    BatchImg& m = P->img[i];
     const split_hip_image_desc& e = d[i];
     if (e.width < 1 || e.height < 1 || (int64_t)e.width * e.height > INT32_MAX)
       return fail(SPLIT_HIP_ERR_ARGUMENT, "image %d is %d x %d", i, e.width,
@@ -440,9 +573,27 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
     if (e.k < 1 || e.k > kBatchMaxK)
       return fail(SPLIT_HIP_ERR_UNSUPPORTED, "image %d: k = %d (batch k <= %d)",
                   i, e.k, kBatchMaxK);
-    if (t && (!e.points || !e.centroids || !e.labels || !e.segments))
+    if (temp && (!e.points || !e.centroids || !e.labels || !e.segments))
       return fail(SPLIT_HIP_ERR_ARGUMENT, "image %d has a null buffer", i);
+  }
+  const bool can_bound = max_iter <= kDmSlots - 2;
+  auto bounded = [&](int i) { return can_bound && d[i].k >= kBoundedMinK; };
+  P->order.clear();
+  for (int i = 0; i < count; ++i)
+    if (!bounded(i)) P->order.push_back(i);
+  P->nplain = (int)P->order.size();
+  for (int i = 0; i < count; ++i)
+    if (bounded(i)) P->order.push_back(i);
+
+  P->img.assign(count, BatchImg{});
+  char* t = static_cast<char*>(temp);
+  size_t o = align_up((size_t)count * sizeof(BatchImg), 256);
+  int64_t tiles = 0, ctiles = 0, px = 0;
+  for (int j = 0; j < count; ++j) {
+    BatchImg& m = P->img[j];
+    const split_hip_image_desc& e = d[P->order[j]];
     const int64_t n = (int64_t)e.width * e.height;
+    if (j == P->nplain) P->tiles_plain = tiles;
     m.pts = e.points;
     m.labels = e.labels;
     m.segments = e.segments;
@@ -451,6 +602,7 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
     m.W = e.width;
     m.H = e.height;
     m.k = e.k;
+    m.bounded = j >= P->nplain;
     m.tiles_x = (e.width + kTileSide - 1) / kTileSide;
     m.ctiles_x = (e.width + kCcTile - 1) / kCcTile;
     m.tile0 = tiles;
@@ -468,7 +620,10 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
     m.totals = reinterpret_cast<long long*>(take(4 * (size_t)e.k * 8));
     m.bitmap = reinterpret_cast<unsigned*>(take(2 * (size_t)((n + 63) / 64) * 4));
     m.prefix = reinterpret_cast<unsigned*>(take((size_t)((n + 31) / 32) * 4));
+    m.pstate = reinterpret_cast<uint32_t*>(take((size_t)n * 4));
+    m.dm = reinterpret_cast<float*>(take(kDmSlots * sizeof(float)));
   }
+  if (P->nplain == count) P->tiles_plain = tiles;
   P->tiles = tiles;
   P->ctiles = ctiles;
   P->px = px;
@@ -477,34 +632,51 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
 }
 
 struct StepTune {
-  int threads = 1024;
+  int threads = 512;
   int per_cu = 1;
 };
 
-static size_t step_lds(int threads) {
-  return (size_t)(threads / kWave) * sizeof(WaveLds);
+static size_t step_lds(bool bounded, int threads) {
+  return (size_t)(threads / kWave) *
+         (bounded ? sizeof(WaveLds) : sizeof(WaveLdsPlain));
 }
 
-static void launch_batch_step(const BatchArgs& a, const StepTune& t,
-                              hipStream_t stream) {
+static const void* step_fn(bool bounded, bool first) {
+  if (!bounded) return (const void*)batch_step_plain;
+  return first ? (const void*)batch_step_bounded<true>
+               : (const void*)batch_step_bounded<false>;
+}
+
+static void launch_step(const BatchArgs& a, const Group& g, bool bounded,
+                        bool first, const StepTune& t, hipStream_t stream) {
   const int nwaves = t.threads / kWave;
-  const int64_t want = (a.tiles + nwaves - 1) / nwaves;
+  const int64_t want = (g.t1 - g.t0 + nwaves - 1) / nwaves;
   const int grid = (int)std::max<int64_t>(
       1, std::min<int64_t>(want, (int64_t)device_cu_count() * t.per_cu));
-  hipLaunchKernelGGL(batch_step, dim3(grid), dim3(t.threads), step_lds(t.threads),
-                     stream, a);
+  const size_t lds = step_lds(bounded, t.threads);
+  if (!bounded)
+    hipLaunchKernelGGL(batch_step_plain, dim3(grid), dim3(t.threads), lds,
+                       stream, a, g);
+  else if (first)
+    hipLaunchKernelGGL(batch_step_bounded<true>, dim3(grid), dim3(t.threads),
+                       lds, stream, a, g);
+  else
+    hipLaunchKernelGGL(batch_step_bounded<false>, dim3(grid), dim3(t.threads),
+                       lds, stream, a, g);
 }
 
-// Autotune cache: batch signature -> best step configuration.
-using TuneKey = std::tuple<int, int, int64_t, int64_t, int>;
+// Autotune cache: (device, group kind, images, tiles, largest k) -> best
+// step configuration.
+using TuneKey = std::tuple<int, int, int, int64_t, int>;
 static std::mutex g_tune_mu;
 static std::map<TuneKey, StepTune> g_tune;
 
-static int autotune_step(const BatchArgs& a, const BatchPlan& P, int device,
-                         hipStream_t stream, StepTune* out) {
-  int kmax = 0;
-  for (const auto& m : P.img) kmax = std::max(kmax, m.k);
-  const TuneKey key{device, a.count, a.tiles, a.px, kmax};
+// Times 3 full-assignment iterations of the group for each block size and
+// blocks-per-CU candidate (up to the occupancy limit).
+static int autotune_group(const BatchArgs& a, const Group& g, bool bounded,
+                          int kmax, int device, hipStream_t stream,
+                          StepTune* out) {
+  const TuneKey key{device, bounded ? 1 : 0, g.count, g.t1 - g.t0, kmax};
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tune.find(key);
@@ -521,28 +693,33 @@ static int autotune_step(const BatchArgs& a, const BatchPlan& P, int device,
   BatchArgs ta = a;
   ta.max_iter = 1 << 30;  // never stop while timing
   ta.threshold = -1.f;
-  for (int threads : {256, 512, 1024}) {
+  const int cands_plain[] = {256, 512, 1024};
+  const int cands_bounded[] = {256, 512};
+  const int* cands = bounded ? cands_bounded : cands_plain;
+  const int ncand = bounded ? 2 : 3;
+  for (int c = 0; c < ncand; ++c) {
+    const int threads = cands[c];
     int occ = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occ, (const void*)batch_step, threads, step_lds(threads)) !=
-            hipSuccess ||
+            &occ, step_fn(bounded, true), threads,
+            step_lds(bounded, threads)) != hipSuccess ||
         occ < 1)
       occ = 1;
     for (int per_cu = 1; per_cu <= occ; per_cu *= 2) {
-      const StepTune c{threads, per_cu};
+      const StepTune cand{threads, per_cu};
       hipLaunchKernelGGL(batch_init, dim3(a.count), dim3(256), 0, stream, ta);
       hipLaunchKernelGGL(batch_absmax, dim3(8 * device_cu_count()), dim3(256),
                          0, stream, ta);
-      launch_batch_step(ta, c, stream);  // warm-up
+      launch_step(ta, g, bounded, true, cand, stream);  // warm-up
       SPLIT_HIP_CHECK(hipEventRecord(e0, stream));
-      for (int r = 0; r < 3; ++r) launch_batch_step(ta, c, stream);
+      for (int r = 0; r < 3; ++r) launch_step(ta, g, bounded, true, cand, stream);
       SPLIT_HIP_CHECK(hipEventRecord(e1, stream));
       SPLIT_HIP_CHECK(hipEventSynchronize(e1));
       float ms = 0.f;
       SPLIT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
       if (ms < best_ms) {
         best_ms = ms;
-        best = c;
+        best = cand;
       }
     }
   }
@@ -557,14 +734,27 @@ static int autotune_step(const BatchArgs& a, const BatchPlan& P, int device,
   return SPLIT_HIP_OK;
 }
 
+struct Groups {
+  Group plain, bounded;
+  StepTune tp, tb;
+};
+
 // Enqueues the whole batched segmentation on `stream`.
-static void enqueue_batch(const BatchArgs& a, const StepTune& t, int conn,
+static void enqueue_batch(const BatchArgs& a, const Groups& G, int conn,
                           int compress, hipStream_t stream) {
   const int cus = device_cu_count();
   hipLaunchKernelGGL(batch_init, dim3(a.count), dim3(256), 0, stream, a);
   hipLaunchKernelGGL(batch_absmax, dim3(8 * cus), dim3(256), 0, stream, a);
-  for (int it = 0; it < a.max_iter; ++it) launch_batch_step(a, t, stream);
-  hipLaunchKernelGGL(batch_labels, dim3(8 * cus), dim3(256), 0, stream, a);
+  for (int it = 0; it < a.max_iter; ++it) {
+    if (G.plain.count) launch_step(a, G.plain, false, it == 0, G.tp, stream);
+    if (G.bounded.count) launch_step(a, G.bounded, true, it == 0, G.tb, stream);
+  }
+  if (G.plain.count)
+    hipLaunchKernelGGL(batch_labels_plain, dim3(8 * cus), dim3(256), 0, stream,
+                       G.plain);
+  if (G.bounded.count)
+    hipLaunchKernelGGL(batch_labels_state, dim3(8 * cus), dim3(256), 0, stream,
+                       G.bounded);
   hipLaunchKernelGGL(batch_finish, dim3(a.count), dim3(256), 0, stream, a);
   const int cgrid = (int)std::min<int64_t>(a.ctiles, (int64_t)cus * 16);
   if (conn == 8) {
@@ -590,8 +780,10 @@ struct GraphEntry {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
-using GraphKey = std::tuple<int, void*, int, int64_t, int64_t, int, float, int,
-                            int, int, int>;
+// (device, temp, images, plain images, plain tiles, tiles, pixels, max_iter,
+//  threshold, connectivity, compress, tuned plain / bounded shapes)
+using GraphKey = std::tuple<int, void*, int, int, int64_t, int64_t, int64_t,
+                            int, float, int, int, int, int, int, int>;
 static std::mutex g_graph_mu;
 static std::map<GraphKey, GraphEntry> g_graphs;
 
@@ -605,7 +797,7 @@ size_t split_hip_segment_batch_workspace(const split_hip_image_desc* images,
                                          int32_t count) {
   if (!images || count < 1) return 0;
   BatchPlan P;
-  if (make_plan(images, count, nullptr, &P) != SPLIT_HIP_OK) return 0;
+  if (make_plan(images, count, nullptr, 0, &P) != SPLIT_HIP_OK) return 0;
   return P.total_bytes;
 }
 
@@ -620,7 +812,7 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
   if (connectivity != 4 && connectivity != 8)
     return fail(SPLIT_HIP_ERR_ARGUMENT, "connectivity %d", connectivity);
   BatchPlan P;
-  SPLIT_HIP_TRY(make_plan(images, count, temp, &P));
+  SPLIT_HIP_TRY(make_plan(images, count, temp, max_iter, &P));
   if (temp_bytes < P.total_bytes)
     return fail(SPLIT_HIP_ERR_TEMP, "temp holds %zu bytes, batch needs %zu",
                 temp_bytes, P.total_bytes);
@@ -637,13 +829,22 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
                                  hipMemcpyHostToDevice, stream));
   int device = 0;
   SPLIT_HIP_CHECK(hipGetDevice(&device));
-  StepTune tune;
-  SPLIT_HIP_TRY(autotune_step(a, P, device, stream, &tune));
+  Groups G;
+  G.plain = Group{a.img, P.nplain, 0, P.tiles_plain};
+  G.bounded = Group{a.img + P.nplain, count - P.nplain, P.tiles_plain, P.tiles};
+  int kp = 0, kb = 0;
+  for (int j = 0; j < count; ++j)
+    (j < P.nplain ? kp : kb) = std::max(j < P.nplain ? kp : kb, P.img[j].k);
+  if (G.plain.count)
+    SPLIT_HIP_TRY(autotune_group(a, G.plain, false, kp, device, stream, &G.tp));
+  if (G.bounded.count)
+    SPLIT_HIP_TRY(autotune_group(a, G.bounded, true, kb, device, stream, &G.tb));
 
   if (use_graph) {
-    const GraphKey key{device, temp, count, P.tiles, P.px, a.max_iter,
-                       threshold, connectivity, compress != 0, tune.threads,
-                       tune.per_cu};
+    const GraphKey key{device, temp, count, P.nplain, P.tiles_plain, P.tiles,
+                       P.px, a.max_iter, threshold, connectivity,
+                       compress != 0, G.tp.threads, G.tp.per_cu,
+                       G.tb.threads, G.tb.per_cu};
     hipGraphExec_t exec = nullptr;
     {
       std::lock_guard<std::mutex> lock(g_graph_mu);
@@ -655,7 +856,7 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
       SPLIT_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
       GraphEntry e;
       SPLIT_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-      enqueue_batch(a, tune, connectivity, compress != 0, cap);
+      enqueue_batch(a, G, connectivity, compress != 0, cap);
       const hipError_t le = hipGetLastError();
       SPLIT_HIP_CHECK(hipStreamEndCapture(cap, &e.graph));
       if (le != hipSuccess)
@@ -669,7 +870,7 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
     }
     SPLIT_HIP_CHECK(hipGraphLaunch(exec, stream));
   } else {
-    enqueue_batch(a, tune, connectivity, compress != 0, stream);
+    enqueue_batch(a, G, connectivity, compress != 0, stream);
     SPLIT_HIP_CHECK_LAUNCH("batch kernels");
   }
   if (results) {
@@ -678,10 +879,11 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
                                    (size_t)count * sizeof(BatchImg),
                                    hipMemcpyDeviceToHost, stream));
     SPLIT_HIP_CHECK(hipStreamSynchronize(stream));
-    for (int i = 0; i < count; ++i) {
-      results[i].iterations = back[i].iter;
-      results[i].shift = back[i].shift;
-      results[i].num_segments = compress ? back[i].nseg : -1;
+    for (int j = 0; j < count; ++j) {
+      split_hip_batch_result& r = results[P.order[j]];
+      r.iterations = back[j].iter;
+      r.shift = back[j].shift;
+      r.num_segments = compress ? back[j].nseg : -1;
     }
   } else {
     SPLIT_HIP_CHECK(hipStreamSynchronize(stream));

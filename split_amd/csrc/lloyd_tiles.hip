@@ -32,39 +32,9 @@
 #include "common.hpp"
 #include "lloyd_device.hpp"
 #include "lloyd_sums.hpp"
+#include "lloyd_bounded.hpp"
 
 namespace splithip {
-
-constexpr int kDmSlots = 256;    // drift history entries; t0 is 8 bits
-
-struct TileArgs : Geometry {
-  int k;
-  float* cbuf;                // two 3k centroid buffers
-  long long* totals;          // [4][k]: quantised channel sums, counts
-  float* dm;                  // cumulative max drift, kDmSlots entries
-  uint32_t* pstate;           // per-pixel state (see above)
-  const unsigned* maxabs;     // bits of max |v| (non-negative float)
-  LloydState* state;
-  Mailbox* mbox;
-};
-
-// fp16 bits of s >= 0 rounded down (to the next smaller half when the
-// nearest one is above s; a value past the fp16 range becomes 65504).
-__device__ __forceinline__ unsigned short half_bits_down(float s) {
-  const _Float16 h = (_Float16)s;
-  unsigned short b = __builtin_bit_cast(unsigned short, h);
-  if ((float)h > s && b > 0) b -= 1;
-  return b;
-}
-__device__ __forceinline__ float half_bits_to_float(unsigned short b) {
-  return (float)__builtin_bit_cast(_Float16, b);
-}
-
-__device__ __forceinline__ uint32_t pack_state(int label, int t0, float slack) {
-  const unsigned short h = half_bits_down(slack);
-  return (uint32_t)(label & 0xFF) | ((uint32_t)(t0 & 0xFF) << 8) |
-         ((uint32_t)h << 16);
-}
 
 // Last block: totals -> centroids, shift, drift history, stop decision.
 __device__ void tiles_finalize(const TileArgs& a, const float* cent,
@@ -130,76 +100,6 @@ __device__ void tiles_finalize(const TileArgs& a, const float* cent,
   }
 }
 
-// Re-assigns up to 256 pixels (4 per lane: idx/v) whose previous labels are
-// old[] (-1: none), writes their new state, and accumulates: every pixel
-// (FULL) or the +/- changes of label (bounded iterations).
-template <int CPL, bool FULL>
-__device__ __forceinline__ void reassign(
-    const TileArgs& a, const float (&cx)[CPL], const float (&cy)[CPL],
-    const float (&cz)[CPL], const float (&cn)[CPL], int it, float fscale,
-    int lane, const int64_t (&idx)[4], const bool (&v)[4], bool lvec,
-    const int (&old)[4], long long* slice, IRun& run) {
-  const int k = a.k;
-  float x[4], y[4], z[4];
-  load_colours(a, idx, v, lvec, x, y, z);
-  int lab[4];
-  float slack[4];
-  assign_segment<CPL, true>(cx, cy, cz, cn, k, lane, x, y, z, v, lab, slack);
-  uint32_t ns[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) ns[i] = pack_state(lab[i], it, slack[i]);
-  if (lvec) {
-    *reinterpret_cast<uint4*>(a.pstate + idx[0]) =
-        make_uint4(ns[0], ns[1], ns[2], ns[3]);
-  } else {
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (v[i]) a.pstate[idx[i]] = ns[i];
-  }
-  int qx[4], qy[4], qz[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    qx[i] = (int)(x[i] * fscale);
-    qy[i] = (int)(y[i] * fscale);
-    qz[i] = (int)(z[i] * fscale);
-  }
-  if constexpr (FULL) {
-    const int l0 = readfirstlane_i(lab[0]);
-    const bool same = lab[0] == l0 && lab[1] == l0 && lab[2] == l0 &&
-                      lab[3] == l0;
-    if (l0 >= 0 && ballot(!same) == 0) {
-      if (l0 != run.label || run.tiles >= kRunFlushTiles) {
-        irun_flush(run, slice, k, lane);
-        run.label = l0;
-      }
-      run.x += (qx[0] + qx[1]) + (qx[2] + qx[3]);
-      run.y += (qy[0] + qy[1]) + (qy[2] + qy[3]);
-      run.z += (qz[0] + qz[1]) + (qz[2] + qz[3]);
-      run.n += 4;
-      run.tiles += 1;
-    } else {
-      add_by_label(slice, k, lane, lab, qx, qy, qz, 1);
-    }
-  } else {
-    bool any = false;
-    int to[4], from[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const bool chg = v[i] && lab[i] != old[i];
-      to[i] = chg ? lab[i] : -1;
-      from[i] = chg ? old[i] : -1;
-      any = any || chg;
-    }
-    if (ballot(any) != 0) {
-      add_by_label(slice, k, lane, to, qx, qy, qz, 1);
-      add_by_label(slice, k, lane, from, qx, qy, qz, -1);
-    }
-  }
-}
-
-constexpr int kQueueCap = 512;      // per-wave queue of failing pixels
-constexpr int kWholeTileFails = 96;  // more failing pixels: re-assign the tile
-
 template <int CPL, int MODE, bool FULL>
 __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
   extern __shared__ long long lds64[];
@@ -246,90 +146,8 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
   const int64_t s0 = a.nseg * gw / W;
   const int64_t s1 = a.nseg * (gw + 1) / W;
 
-  IRun run{0, 0, 0, 0, -1, 0};
-  int qlen = 0;  // wave-uniform
-  // Re-assigns the first min(qlen, 256) queued pixels, then moves the rest
-  // to the front.
-  auto drain = [&]() {
-    const int c = qlen < 256 ? qlen : 256;
-    int64_t idx[4];
-    bool v[4];
-    int old[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int e = lane + 64 * i;
-      v[i] = e < c;
-      idx[i] = qidx[v[i] ? e : 0];
-      old[i] = v[i] ? qold[e] : -1;
-    }
-    reassign<CPL, false>(a, cx, cy, cz, cn, it, fscale, lane, idx, v, false,
-                         old, slice, run);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int e = 256 + lane + 64 * i;
-      if (e < qlen) {
-        qidx[e - 256] = qidx[e];
-        qold[e - 256] = qold[e];
-      }
-    }
-    qlen -= c;
-  };
-
-  for (int64_t s = s0; s < s1; ++s) {
-    int64_t idx[4];
-    bool v[4], lvec;
-    segment_index<MODE>(a, s, lane, idx, v, lvec);
-    if constexpr (FULL) {
-      const int old[4] = {-1, -1, -1, -1};
-      reassign<CPL, true>(a, cx, cy, cz, cn, it, fscale, lane, idx, v, lvec,
-                          old, slice, run);
-    } else {
-      uint32_t ps[4];
-      if (lvec) {
-        const uint4 q = *reinterpret_cast<const uint4*>(a.pstate + idx[0]);
-        ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) ps[i] = v[i] ? a.pstate[idx[i]] : 0u;
-      }
-      int old[4];
-      bool fail[4];
-      uint64_t fb[4];
-      int nfail = 0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        old[i] = v[i] ? (int)(ps[i] & 0xFFu) : -1;
-        const int t0 = (int)((ps[i] >> 8) & 0xFFu);
-        const float sl = half_bits_to_float((unsigned short)(ps[i] >> 16));
-        fail[i] = v[i] && !(sl - 2.f * (drift_now - dmt[t0]) > margin);
-        fb[i] = ballot(fail[i]);
-        nfail += __builtin_popcountll(fb[i]);
-      }
-      if (nfail == 0) continue;  // every pixel provably unchanged
-      if (nfail > kWholeTileFails) {
-        reassign<CPL, false>(a, cx, cy, cz, cn, it, fscale, lane, idx, v, lvec,
-                             old, slice, run);
-        continue;
-      }
-      // queue the failing pixels (in lane order within each slot)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(fb[i] >> 32),
-            __builtin_amdgcn_mbcnt_lo((uint32_t)fb[i], 0u));
-        if (fail[i]) {
-          qidx[qlen + below] = (int)idx[i];
-          qold[qlen + below] = old[i];
-        }
-        qlen += __builtin_popcountll(fb[i]);
-      }
-      if (qlen >= 256) drain();
-    }
-  }
-  if constexpr (!FULL) {
-    while (qlen > 0) drain();
-  }
-  if constexpr (FULL) irun_flush(run, slice, k, lane);
+  walk_tiles<CPL, MODE, FULL>(a, s0, s1, cx, cy, cz, cn, it, fscale, margin,
+                              drift_now, dmt, slice, qidx, qold, lane);
   __syncthreads();
   for (int vv = threadIdx.x; vv < V; vv += blockDim.x) {
     long long tot = 0;
