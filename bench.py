@@ -220,6 +220,52 @@ def batch_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def cc_bench(args):
+    """Connected components (8-connectivity) + compress_labels of the
+    8192x8192 spiral/checkerboard stress label image."""
+    import torch
+    import split_amd as sa
+    from split_amd import workloads as W
+    side = 8192
+    log(f"bench: generating the {side}x{side} spiral/checkerboard labels")
+    lab = torch.from_numpy(W.make_cc_stress(side, side)).cuda()
+    out = torch.empty(side * side, dtype=torch.int32, device="cuda")
+    cc_temp = torch.empty(max(1, sa.lib().split_hip_connected_components_workspace(side, side)),
+                          dtype=torch.uint8, device="cuda")
+    cp_temp = torch.empty(sa.lib().split_hip_compress_labels_workspace(side * side,
+                                                                       side * side),
+                          dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        sa.connected_components(lab, out, connectivity=8, temp=cc_temp)
+        return sa.compress_labels(out, temp=cp_temp)
+
+    for _ in range(args.warmup):
+        nseg = step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        nseg = step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    n = side * side
+    out_line = {
+        "metric": "connected components + compress Mpix/s (8-conn, stress labels)",
+        "value": round(n / (ms / 1e3) / 1e6, 3), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": "cc_stress_8192", "image": f"{side}x{side}",
+                   "connectivity": 8, "segments": nseg, "parallelism": "single GPU"},
+        "algorithmic_bytes_per_step": 16 * n,
+        "hbm_frac": round(16 * n / (ms / 1e3) / (HBM_PEAK_GBS * 1e9), 4),
+    }
+    print(json.dumps(out_line), flush=True)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -267,6 +313,9 @@ def main():
 
     if args.workload == "batch_256":
         batch_bench(args)
+        return
+    if args.workload == "cc_stress_8192":
+        cc_bench(args)
         return
     w = W.WORKLOADS[args.workload]
     if args.profile_driver:

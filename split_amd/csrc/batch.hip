@@ -484,6 +484,20 @@ __global__ __launch_bounds__(3 * kCcTile) void batch_ccl_merge(BatchArgs a) {
   }
 }
 
+// Root jumps (ccl_jump_one) over the padded global pixel space.
+__global__ __launch_bounds__(256) void batch_ccl_jump(BatchArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.px;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t wave0 = __builtin_amdgcn_readfirstlane(
+        (int)((base + threadIdx.x - lane) >> 6)) * (int64_t)64;
+    const int i = find_image<&BatchImg::px0>(a.img, a.count, wave0);
+    const BatchImg& m = a.img[i];
+    const int64_t q = wave0 - m.px0 + lane;
+    if (q < (int64_t)m.W * m.H) ccl_jump_one(m.segments, q, m.W);
+  }
+}
+
 // Flatten + root bitmap over the padded global pixel space (a wave never
 // spans two images, as each image starts at a multiple of 64).
 __global__ __launch_bounds__(256) void batch_flatten(BatchArgs a, int compress) {
@@ -768,6 +782,7 @@ static void enqueue_batch(const BatchArgs& a, const Groups& G, int conn,
     hipLaunchKernelGGL(batch_ccl_merge<4>, dim3(cgrid), dim3(3 * kCcTile), 0,
                        stream, a);
   }
+  hipLaunchKernelGGL(batch_ccl_jump, dim3(8 * cus), dim3(256), 0, stream, a);
   hipLaunchKernelGGL(batch_flatten, dim3(8 * cus), dim3(256), 0, stream, a,
                      compress);
   if (compress) {

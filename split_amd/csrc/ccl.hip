@@ -48,6 +48,12 @@ __global__ __launch_bounds__(3 * kCcTile) void ccl_merge(
 // Every pixel follows its chain to the root.  With `roots` non-null, wave w
 // also writes the presence bitmap of roots for its 64 pixels (bit p set iff
 // pixel p is a root), which is the value set compress_labels needs.
+__global__ void ccl_jump(int32_t* out, int64_t n, int W) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x)
+    ccl_jump_one(out, p, W);
+}
+
 __global__ void ccl_flatten(int32_t* out, int64_t n, uint32_t* roots) {
   const int lane = threadIdx.x & (kWave - 1);
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
@@ -183,7 +189,11 @@ __global__ void compress_mark(const int32_t* v, int64_t n, int vec,
       const bool lead = (i == 0 && first_lane) || before != x[i];
       if (c < chunks && ok[i] && lead) {
         const uint32_t d = (uint32_t)((int64_t)x[i] - lo);
-        atomicOr(&bm[d >> 5], 1u << (d & 31));
+        // test first: values repeat, and an atomic on a hot word serialises
+        const unsigned bit = 1u << (d & 31);
+        if (!(__hip_atomic_load(&bm[d >> 5], __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT) & bit))
+          atomicOr(&bm[d >> 5], bit);
       }
     }
   }
@@ -323,6 +333,8 @@ int connected_components_impl(const int32_t* labels, int W, int H,
   }
   SPLIT_HIP_CHECK_LAUNCH("ccl_tile/ccl_merge");
   const int64_t n = (int64_t)W * H;
+  hipLaunchKernelGGL(ccl_jump, dim3(grid_for(n, 256, 8)), dim3(256), 0, stream,
+                     out, n, W);
   hipLaunchKernelGGL(ccl_flatten, dim3(grid_for(n, 256, 8)), dim3(256), 0,
                      stream, out, n, roots);
   SPLIT_HIP_CHECK_LAUNCH("ccl_flatten");

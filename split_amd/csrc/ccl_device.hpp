@@ -186,6 +186,35 @@ __device__ __forceinline__ void ccl_merge_body(const int32_t* __restrict__ label
 }
 
 
+// Shortens the global forest before the flatten pass.  After ccl_tile and
+// ccl_merge a pixel's parent is its tile's local root (inside the tile)
+// unless the pixel is such a root, whose parent chain crosses tiles and can
+// be as long as the component is winding.  Pixels whose parent lies in
+// another tile, and roots, walk their chain with path halving (every write
+// replaces a parent by an ancestor, so the forest stays valid under
+// concurrent walks) and then point straight at the root they found.
+__device__ __forceinline__ void ccl_jump_one(int32_t* out, int64_t p, int W) {
+  const int q = out[p];
+  if (q == p) return;
+  const int py = (int)(p / W), px = (int)(p - (int64_t)py * W);
+  const int qy = q / W, qx = q - qy * W;
+  if ((py / kCcTile) == (qy / kCcTile) && (px / kCcTile) == (qx / kCcTile))
+    return;  // an in-tile parent: its local root does the walk
+  int x = q;
+  for (;;) {
+    const int px2 = out[x];
+    if (px2 == x) break;
+    const int gp = out[px2];
+    if (gp == px2) {
+      x = px2;
+      break;
+    }
+    out[x] = gp;
+    x = gp;
+  }
+  out[p] = x;
+}
+
 // Follows the chain of pixel p to its root and writes it back.
 __device__ __forceinline__ int ccl_flatten_one(int32_t* out, int64_t p) {
   const int r0 = out[p];
