@@ -266,6 +266,55 @@ def cc_bench(args):
     print(json.dumps(out_line), flush=True)
 
 
+def segmeans_bench(args):
+    """Segment means of the shaded_4096 components: calculate_centroids with
+    k = number of segments, then propagate_centroids (the consumer of CC +
+    compress in the reference's sample)."""
+    import torch
+    import split_amd as sa
+    from split_amd import workloads as W
+    w = W.SHADED_4096
+    P = torch.from_numpy(W.make_image(w.image)).cuda()
+    C = torch.empty((3, w.k), dtype=torch.float32, device="cuda")
+    L = torch.empty(w.n, dtype=torch.int32, device="cuda")
+    S = torch.empty(w.n, dtype=torch.int32, device="cuda")
+    sa.uniform_random_initialize(P, C, w.init_seed)
+    r = sa.segment(P, 4096, 4096, C, L, S, max_iter=w.max_iter, threshold=w.threshold,
+                   connectivity=w.connectivity)
+    k = r.num_segments
+    M = torch.empty((3, k), dtype=torch.float32, device="cuda")
+    out = torch.empty_like(P)
+    temp = torch.empty(sa.lib().split_hip_calculate_centroids_workspace(w.n, k),
+                       dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        sa.calculate_centroids(S, P, M, temp=temp)
+        sa.propagate_centroids(S, M, out)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    print(json.dumps({
+        "metric": "segment means + propagation Mpix/s",
+        "value": round(w.n / (ms / 1e3) / 1e6, 3), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": "segmeans_4096", "image": "4096x4096", "segments": k,
+                   "parallelism": "single GPU"},
+        "algorithmic_bytes_per_step": 32 * w.n,
+        "hbm_frac": round(32 * w.n / (ms / 1e3) / (HBM_PEAK_GBS * 1e9), 4),
+    }), flush=True)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -316,6 +365,9 @@ def main():
         return
     if args.workload == "cc_stress_8192":
         cc_bench(args)
+        return
+    if args.workload == "segmeans_4096":
+        segmeans_bench(args)
         return
     w = W.WORKLOADS[args.workload]
     if args.profile_driver:

@@ -1,5 +1,4 @@
-// Drop-in for split/include/split/device/kmeans/kmeans.cuh:4-8
-// (propagate_centroids stays on its existing path and is not included).
+// Drop-in for split/include/split/device/kmeans/kmeans.cuh:4-8.
 #if !defined(SPLIT_DEVICE_INCLUDED_KMEANS)
 #define SPLIT_DEVICE_INCLUDED_KMEANS
 
@@ -7,5 +6,6 @@
 #include "split/device/kmeans/centroids.cuh"
 #include "split/device/kmeans/label.cuh"
 #include "split/device/kmeans/cluster.cuh"
+#include "split/device/kmeans/propagation.cuh"
 
 #endif  // SPLIT_DEVICE_INCLUDED_KMEANS

@@ -93,11 +93,25 @@ size_t split_hip_calculate_centroids_workspace(int64_t n, int32_t k);
  * centroids[c*k+j] = mean of channel c over pixels labelled j; a cluster with
  * no pixels gets the zero centroid (the reference's stated intent,
  * centroids.cu:65-67).  Labels outside [0, k) contribute nothing.
- * Counts are exact integers; sums are float, reduced in a fixed order. */
+ * Counts are exact integers.  k <= SPLIT_HIP_MAX_K: float sums reduced in a
+ * fixed order.  Larger k (segment means, k = number of segments, the
+ * caller's use at samples/src/main.cu:255-307): sums of channels quantised
+ * to 2^-(22-e) (max|v| <= 2^e) accumulated exactly in int64, so the result
+ * does not depend on order; the temp needed is about 28 bytes per label. */
 int split_hip_calculate_centroids(const int32_t* labels, const float* points,
                                   int64_t n, float* centroids, int32_t k,
                                   void* temp, size_t temp_bytes,
                                   split_hip_stream_t stream);
+
+/* Replaces split::device::kmeans::propagate_centroids
+ * (split/include/split/device/kmeans/propagation.cuh:24-27,
+ *  split/src/split/device/kmeans/propagation.cu:9-24): points[c*n+i] =
+ * centroids[c*k + labels[i]] for every pixel (3 x n SoA planes).  Pixels
+ * whose label is outside [0, k) keep their values (the reference gathers
+ * out of range).  Any k. */
+int split_hip_propagate_centroids(const int32_t* labels, int64_t n,
+                                  const float* centroids, int64_t k,
+                                  float* points, split_hip_stream_t stream);
 
 /* Replaces split::device::kmeans::cluster_workspace
  * (split/include/split/device/kmeans/cluster.cuh:46-47,

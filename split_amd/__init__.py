@@ -101,6 +101,8 @@ CABI = {
     "split_hip_calculate_centroids_workspace": (_c_size_t, [_c_i64, _c_i32]),
     "split_hip_calculate_centroids": (ctypes.c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p,
                                                      _c_i32, _c_void_p, _c_size_t, _c_void_p]),
+    "split_hip_propagate_centroids": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p, _c_i64,
+                                                     _c_void_p, _c_void_p]),
     "split_hip_cluster_workspace": (_c_size_t, [_c_i64, _c_i32, _c_i32]),
     "split_hip_cluster": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p, _c_i32, _c_void_p,
                                          _c_void_p, _c_size_t, _c_i32, _c_float,
@@ -259,6 +261,19 @@ def calculate_centroids(labels, points, centroids, temp=None) -> None:
                                            _stream()), "calculate_centroids")
 
 
+def propagate_centroids(labels, centroids, points) -> None:
+    """split::device::kmeans::propagate_centroids (propagation.cu:9-24):
+    points[:, i] = centroids[:, labels[i]] (labels outside [0, K) untouched)."""
+    torch = _torch()
+    p, n = _points(points)
+    c, k = _cent(centroids)
+    lb = _dev(labels, torch.int32, "labels")
+    if labels.numel() != n:
+        raise ValueError("labels must hold N entries")
+    _check(lib().split_hip_propagate_centroids(lb, n, c, k, p, _stream()),
+           "propagate_centroids")
+
+
 def cluster(points, centroids, labels, max_iter: int, threshold: float = 1.0,
             temp=None, *, width: int = 0, height: int = 0) -> tuple[int, float]:
     """split::device::kmeans::cluster (cluster.cu:27-93).  Returns
@@ -296,7 +311,8 @@ def uniform_random_initialize(points, centroids, seed: int) -> None:
 
 
 kmeans = SimpleNamespace(label_points=label_points, calculate_centroids=calculate_centroids,
-                         cluster=cluster, uniform_random_initialize=uniform_random_initialize)
+                         cluster=cluster, uniform_random_initialize=uniform_random_initialize,
+                         propagate_centroids=propagate_centroids)
 
 
 # ------------------------------------------------------------------- ccl ---

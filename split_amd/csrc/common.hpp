@@ -153,6 +153,15 @@ struct DriverRes {
 int driver_res(DriverRes** out);
 
 // Bounded tiled Lloyd (lloyd_tiles.hip).
+// Largest |v| of count floats as float bits into *out (which must start at
+// 0), stream-ordered.
+int launch_absmax(const float* pts, int64_t count, unsigned* out,
+                  hipStream_t stream);
+// calculate_centroids for any k (segment means; exact integer sums).
+size_t segment_means_workspace(int64_t k);
+int segment_means(const int32_t* labels, const float* points, int64_t n,
+                  float* centroids, int64_t k, void* temp, size_t temp_bytes,
+                  hipStream_t stream);
 size_t tiles_workspace(int64_t n, int k);
 bool tiles_supported(int width, int k, int max_iter);
 int cluster_tiles_async(const float* pts, int width, int height,

@@ -672,6 +672,7 @@ int split_hip_label_image(const float* centroids, int32_t k,
 
 size_t split_hip_calculate_centroids_workspace(int64_t, int32_t k) {
   if (k < 1) k = 1;
+  if (k > SPLIT_HIP_MAX_K) return segment_means_workspace(k);
   return kStateBytes + partial_bytes(k);
 }
 
@@ -679,6 +680,13 @@ int split_hip_calculate_centroids(const int32_t* labels, const float* points,
                                   int64_t n, float* centroids, int32_t k,
                                   void* temp, size_t temp_bytes,
                                   split_hip_stream_t stream) {
+  if (k > SPLIT_HIP_MAX_K) {  // segment means: k beyond the register kernels
+    if (!points || !labels || !centroids || !temp || n < 1)
+      return fail(SPLIT_HIP_ERR_ARGUMENT,
+                  "points, labels, centroids or temp is null, or n < 1");
+    return segment_means(labels, points, n, centroids, k, temp, temp_bytes,
+                         stream);
+  }
   SPLIT_HIP_TRY(check_points(points, n, k));
   if (!labels || !centroids || !temp)
     return fail(SPLIT_HIP_ERR_ARGUMENT, "labels, centroids or temp is null");

@@ -265,6 +265,15 @@ static TileLayout tiles_layout(void* temp, int64_t n, int k) {
   return L;
 }
 
+int launch_absmax(const float* pts, int64_t count, unsigned* out,
+                  hipStream_t stream) {
+  const int vec = ((uintptr_t)pts & 15u) == 0;
+  hipLaunchKernelGGL(tiles_absmax, dim3(8 * device_cu_count()), dim3(256), 0,
+                     stream, pts, count, vec, out);
+  SPLIT_HIP_CHECK_LAUNCH("absmax");
+  return SPLIT_HIP_OK;
+}
+
 size_t tiles_workspace(int64_t n, int k) {
   return tiles_layout(nullptr, n, k).bytes;
 }

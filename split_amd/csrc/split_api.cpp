@@ -101,6 +101,21 @@ void calculate_centroids(const_view1d<int> di_labels,
   sync("calculate_centroids");
 }
 
+// split/src/split/device/kmeans/propagation.cu:9-24
+void propagate_centroids(const_view1d<int> di_cluster_labels,
+                         const_view2d<real> di_centroids,
+                         view2d<real> do_points)
+{
+  rows3(di_centroids.num_rows, "propagate_centroids");
+  rows3(do_points.num_rows, "propagate_centroids");
+  check(split_hip_propagate_centroids(
+            di_cluster_labels.data(), (std::int64_t)do_points.num_cols,
+            di_centroids.values.data(), (std::int64_t)di_centroids.num_cols,
+            do_points.values.data(), nullptr),
+        "propagate_centroids");
+  sync("propagate_centroids");
+}
+
 // split/src/split/device/kmeans/cluster.cu:12-25
 std::size_t cluster_workspace(const int i_npoints, const int i_nclusters,
                               const int i_dim)

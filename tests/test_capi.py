@@ -43,6 +43,9 @@ def test_host_only_entry_points_without_gpu():
     n = 4096 * 4096
     assert 4 * n <= ws < 4 * n + (1 << 20)  # per-pixel Lloyd state + small tables
     assert lib.split_hip_compress_labels_workspace(1 << 20, 1 << 20) < 4 << 20
+    # segment means (k beyond the register kernels): ~28 bytes per label
+    ws = lib.split_hip_calculate_centroids_workspace(1 << 20, 200000)
+    assert 28 * 200000 <= ws < 28 * 200000 + 4096
     # batch workspace: sizes only (buffers may be null), k limited to 64
     descs = (sa.ImageDesc * 3)()
     for i, (w, h, k) in enumerate([(2048, 2048, 64), (37, 53, 4), (1, 1, 1)]):
