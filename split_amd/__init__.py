@@ -329,8 +329,12 @@ def connected_components(labels_hw, out, connectivity: int = 4, max_iterations: 
     oi = _dev(out, torch.int32, "out")
     if out.numel() != h * w:
         raise ValueError("out must hold H*W entries")
-    _check(lib().split_hip_connected_components(li, w, h, None, 0, oi, int(max_iterations),
-                                                int(connectivity), _stream()),
+    L = lib()
+    if temp is None:
+        temp = _temp(L.split_hip_connected_components_workspace(w, h), out)
+    _check(L.split_hip_connected_components(li, w, h, temp.data_ptr(), temp.numel(), oi,
+                                            int(max_iterations), int(connectivity),
+                                            _stream()),
            "connected_components")
 
 

@@ -12,12 +12,13 @@
 //    and slack, later ones skip the pixels whose bound proves the label
 //    unchanged and re-assign the rest; the labels come from that state.
 // A wave walks a contiguous run of its group's tiles, switching image
-// (centroids, k, scale) where the run crosses an image boundary.  Sums are
+// (centroids, k, scale, bound table) where the run crosses an image
+// boundary.  Sums are
 // exact integers (quantised as in the single-image tiled path), so every
 // image gets exactly the centroids, labels and stopping iteration it gets on
 // its own.  The wave that completes an image's last tile of the iteration
-// (per-image tile counter) finalises that image: new centroids, shift, drift
-// history, stop flag.  Images finish independently; later launches skip
+// (per-image tile counter) finalises that image: new centroids, shift, bound
+// table, stop flag.  Images finish independently; later launches skip
 // their tiles.
 //
 // After the iterations: the labels (from the per-pixel state), connected
@@ -64,7 +65,7 @@ struct BatchImg {
   unsigned* bitmap;      // 2*ceil(n/64) words
   unsigned* prefix;      // ceil(n/32) words
   uint32_t* pstate;      // per-pixel Lloyd state (lloyd_bounded.hpp), n
-  float* dm;             // cumulative max drift, kDmSlots
+  float* fcum;           // per-centroid bound consumption, k
   // device state
   int cur, iter, done, nseg;
   float shift;
@@ -75,6 +76,7 @@ struct BatchImg {
 
 struct BatchArgs {
   BatchImg* img;
+  unsigned* ccl_flag;  // connected-component check failures
   int count;
   int64_t tiles;   // total Lloyd tiles
   int64_t ctiles;  // total CC tiles
@@ -93,8 +95,7 @@ struct Group {
 
 // Bounded iterations pay for reading and writing the per-pixel state and
 // for queueing; they win once the full assignment is expensive (k >= 32 on
-// the shaded workloads) and need the drift history of every set (t0 is 8
-// bits).
+// the shaded workloads).
 constexpr int kBoundedMinK = 32;
 
 // Image owning global index g of the field at byte offset OFF (tile0,
@@ -137,8 +138,8 @@ __global__ void batch_init(BatchArgs a) {
     m.shift = 0.f;
     m.maxabs = 0u;
     m.tiles_done = 0;
-    m.dm[0] = 0.f;
   }
+  for (int i = threadIdx.x; i < k; i += blockDim.x) m.fcum[i] = 0.f;
 }
 
 // Largest |v| per image, one wave per Lloyd tile.
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(256) void batch_absmax(BatchArgs a) {
 }
 
 // Per-wave LDS: accumulation slice [4][kBatchMaxK] of int64 and 64 doubles
-// of finalisation scratch; the bounded kernel adds the drift history and the
+// of finalisation scratch; the bounded kernel adds the bound table and the
 // queue of failing pixels.
 struct WaveLdsPlain {
   long long slice[4 * kBatchMaxK];
@@ -175,7 +176,7 @@ struct WaveLdsPlain {
 struct WaveLds {
   long long slice[4 * kBatchMaxK];
   double scratch[kWave];
-  float dmt[kDmSlots];
+  float fl[kBatchMaxK];
   int qidx[kQueueCap];
   int qold[kQueueCap];
 };
@@ -229,18 +230,27 @@ __device__ void batch_finish_segment(const BatchArgs& a, BatchImg& m,
   // re-assign every pixel start from zero
   if (!m.bounded)
     for (int v = lane; v < 4 * k; v += kWave) m.totals[v] = 0;
+  __builtin_amdgcn_wave_barrier();
+  double m1 = 0.0, m2 = 0.0;  // largest and second-largest squared moves
+  int arg = -1;
+  for (int j = 0; j < k; ++j) {
+    const double d = w.scratch[j];
+    if (d > m1) {
+      m2 = m1;
+      m1 = d;
+      arg = j;
+    } else if (d > m2) {
+      m2 = d;
+    }
+  }
+  if (m.bounded && lane < k)
+    m.fcum[lane] = bound_step(m.fcum[lane], d2, lane, m1, m2, arg);
   __threadfence();
   if (lane == 0) {
-    double s = 0.0, mx = 0.0;
-    for (int j = 0; j < kWave; ++j) {
-      s += w.scratch[j];
-      mx = fmax(mx, w.scratch[j]);
-    }
+    double s = 0.0;
+    for (int j = 0; j < kWave; ++j) s += w.scratch[j];
     const float shift = (float)sqrt(s);
     const int it = m.iter + 1;
-    // rounded up: the skip test must never under-estimate the drift
-    if (it < kDmSlots)
-      m.dm[it] = (float)((double)m.dm[it - 1] + sqrt(mx) * (1.0 + 1e-5) + 1e-7);
     m.shift = shift;
     m.iter = it;
     m.cur = 1 - m.cur;
@@ -357,7 +367,6 @@ __global__ __launch_bounds__(512) void batch_step_bounded(BatchArgs a, Group g) 
       continue;
     }
     const int k = m.k;
-    const int it = m.iter;
     ta.k = k;
     ta.pstate = m.pstate;
     const float* cent = m.cbuf + (size_t)m.cur * 3 * k;
@@ -372,23 +381,17 @@ __global__ __launch_bounds__(512) void batch_step_bounded(BatchArgs a, Group g) 
     const float maxabs = __uint_as_float(m.maxabs);
     const float fscale = scale_of(m.maxabs);
     const float margin = 1e-5f * (1.f + 4.f * maxabs);
-    float drift_now = 0.f;
-    if constexpr (!FIRST) {
-      for (int t = lane; t <= it; t += kWave) w.dmt[t] = m.dm[t];
-      __builtin_amdgcn_wave_barrier();
-      drift_now = w.dmt[it];
-    }
+    if (lane < k) w.fl[lane] = m.fcum[lane];
+    __builtin_amdgcn_wave_barrier();
     const bool vec = (ta.width % 4 == 0) && ((((uintptr_t)ta.pts) & 15u) == 0) &&
                      ((((uintptr_t)ta.pstate) & 15u) == 0);
     const int64_t t0 = s - m.tile0, t1 = end - m.tile0;
     if (vec)
-      walk_tiles<1, kTileVec, FIRST>(ta, t0, t1, cx, cy, cz, cn, it, fscale,
-                                     margin, drift_now, w.dmt, w.slice, w.qidx,
-                                     w.qold, lane);
+      walk_tiles<1, kTileVec, FIRST>(ta, t0, t1, cx, cy, cz, cn, w.fl, fscale,
+                                     margin, w.slice, w.qidx, w.qold, lane);
     else
-      walk_tiles<1, kTile, FIRST>(ta, t0, t1, cx, cy, cz, cn, it, fscale,
-                                  margin, drift_now, w.dmt, w.slice, w.qidx,
-                                  w.qold, lane);
+      walk_tiles<1, kTile, FIRST>(ta, t0, t1, cx, cy, cz, cn, w.fl, fscale,
+                                  margin, w.slice, w.qidx, w.qold, lane);
     batch_finish_segment(a, m, (int)(end - s), w, lane);
     s = end;
   }
@@ -463,6 +466,7 @@ template <int CONN>
 __global__ __launch_bounds__(kCcThreads) void batch_ccl_tile(BatchArgs a) {
   __shared__ int lab[kCcTilePx];
   __shared__ int par[kCcTilePx];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.ccl_flag = 0u;
   for (int64_t c = blockIdx.x; c < a.ctiles; c += gridDim.x) {
     const int i = find_image<&BatchImg::ctile0>(a.img, a.count, c);
     const BatchImg& m = a.img[i];
@@ -498,9 +502,29 @@ __global__ __launch_bounds__(256) void batch_ccl_jump(BatchArgs a) {
   }
 }
 
+// Check of the flattened forests (ccl_check_one) over the padded global
+// pixel space.
+template <int CONN>
+__global__ __launch_bounds__(256) void batch_ccl_check(BatchArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.px;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t wave0 = __builtin_amdgcn_readfirstlane(
+        (int)((base + threadIdx.x - lane) >> 6)) * (int64_t)64;
+    const int i = find_image<&BatchImg::px0>(a.img, a.count, wave0);
+    const BatchImg& m = a.img[i];
+    const int64_t q = wave0 - m.px0 + lane;
+    if (q < (int64_t)m.W * m.H)
+      ccl_check_one<CONN>(m.labels, m.W, m.H, m.segments, q, a.ccl_flag);
+  }
+}
+
 // Flatten + root bitmap over the padded global pixel space (a wave never
-// spans two images, as each image starts at a multiple of 64).
-__global__ __launch_bounds__(256) void batch_flatten(BatchArgs a, int compress) {
+// spans two images, as each image starts at a multiple of 64).  recheck: run
+// only when the check pass found a failure.
+__global__ __launch_bounds__(256) void batch_flatten(BatchArgs a, int compress,
+                                                     int recheck) {
+  if (recheck && *a.ccl_flag == 0u) return;
   const int lane = threadIdx.x & (kWave - 1);
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.px;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -590,8 +614,8 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
     if (temp && (!e.points || !e.centroids || !e.labels || !e.segments))
       return fail(SPLIT_HIP_ERR_ARGUMENT, "image %d has a null buffer", i);
   }
-  const bool can_bound = max_iter <= kDmSlots - 2;
-  auto bounded = [&](int i) { return can_bound && d[i].k >= kBoundedMinK; };
+  (void)max_iter;
+  auto bounded = [&](int i) { return d[i].k >= kBoundedMinK; };
   P->order.clear();
   for (int i = 0; i < count; ++i)
     if (!bounded(i)) P->order.push_back(i);
@@ -601,7 +625,7 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
 
   P->img.assign(count, BatchImg{});
   char* t = static_cast<char*>(temp);
-  size_t o = align_up((size_t)count * sizeof(BatchImg), 256);
+  size_t o = align_up((size_t)count * sizeof(BatchImg), 256) + 256;  // + flag
   int64_t tiles = 0, ctiles = 0, px = 0;
   for (int j = 0; j < count; ++j) {
     BatchImg& m = P->img[j];
@@ -635,7 +659,7 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
     m.bitmap = reinterpret_cast<unsigned*>(take(2 * (size_t)((n + 63) / 64) * 4));
     m.prefix = reinterpret_cast<unsigned*>(take((size_t)((n + 31) / 32) * 4));
     m.pstate = reinterpret_cast<uint32_t*>(take((size_t)n * 4));
-    m.dm = reinterpret_cast<float*>(take(kDmSlots * sizeof(float)));
+    m.fcum = reinterpret_cast<float*>(take((size_t)e.k * sizeof(float)));
   }
   if (P->nplain == count) P->tiles_plain = tiles;
   P->tiles = tiles;
@@ -784,7 +808,15 @@ static void enqueue_batch(const BatchArgs& a, const Groups& G, int conn,
   }
   hipLaunchKernelGGL(batch_ccl_jump, dim3(8 * cus), dim3(256), 0, stream, a);
   hipLaunchKernelGGL(batch_flatten, dim3(8 * cus), dim3(256), 0, stream, a,
-                     compress);
+                     compress, 0);
+  if (conn == 8)
+    hipLaunchKernelGGL(batch_ccl_check<8>, dim3(8 * cus), dim3(256), 0, stream,
+                       a);
+  else
+    hipLaunchKernelGGL(batch_ccl_check<4>, dim3(8 * cus), dim3(256), 0, stream,
+                       a);
+  hipLaunchKernelGGL(batch_flatten, dim3(8 * cus), dim3(256), 0, stream, a,
+                     compress, 1);
   if (compress) {
     hipLaunchKernelGGL(batch_scan, dim3(a.count), dim3(1024), 0, stream, a);
     hipLaunchKernelGGL(batch_map, dim3(8 * cus), dim3(256), 0, stream, a);
@@ -833,6 +865,8 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
                 temp_bytes, P.total_bytes);
   BatchArgs a;
   a.img = static_cast<BatchImg*>(temp);
+  a.ccl_flag = reinterpret_cast<unsigned*>(
+      static_cast<char*>(temp) + align_up((size_t)count * sizeof(BatchImg), 256));
   a.count = count;
   a.tiles = P.tiles;
   a.ctiles = P.ctiles;

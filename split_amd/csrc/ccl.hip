@@ -33,9 +33,11 @@ namespace splithip {
 
 template <int CONN>
 __global__ __launch_bounds__(kCcThreads) void ccl_tile(
-    const int32_t* __restrict__ labels, int W, int H, int32_t* out) {
+    const int32_t* __restrict__ labels, int W, int H, int32_t* out,
+    unsigned* flag) {
   __shared__ int lab[kCcTilePx];
   __shared__ int par[kCcTilePx];
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *flag = 0u;
   ccl_tile_body<CONN>(labels, W, H, out, blockIdx.x, blockIdx.y, lab, par);
 }
 
@@ -54,7 +56,19 @@ __global__ void ccl_jump(int32_t* out, int64_t n, int W) {
     ccl_jump_one(out, p, W);
 }
 
-__global__ void ccl_flatten(int32_t* out, int64_t n, uint32_t* roots) {
+template <int CONN>
+__global__ void ccl_check(const int32_t* __restrict__ labels, int W, int H,
+                          int32_t* out, unsigned* flag) {
+  const int64_t n = (int64_t)W * H;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x)
+    ccl_check_one<CONN>(labels, W, H, out, p, flag);
+}
+
+// flag == null: always; otherwise only when the check pass found a failure.
+__global__ void ccl_flatten(int32_t* out, int64_t n, uint32_t* roots,
+                            const unsigned* flag) {
+  if (flag && *flag == 0u) return;
   const int lane = threadIdx.x & (kWave - 1);
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -78,9 +92,11 @@ struct CompressState {
   int err;          // 1: value span exceeds the bitmap capacity of temp
   int64_t words;    // bitmap words in use
   int64_t total;    // distinct values
+  unsigned ccl_flag;  // connected-component check failures (segment path)
 };
 
 constexpr size_t kCompressHead = 64;
+constexpr size_t kCcTempBytes = 256;  // the check pass's failure flag
 constexpr int kScanThreads = 256;
 constexpr int kScanWordsPerThread = 16;
 constexpr int kScanChunk = kScanThreads * kScanWordsPerThread;  // 4096 words
@@ -305,9 +321,9 @@ static size_t compress_csum_bytes(int64_t words) {
 
 int connected_components_impl(const int32_t* labels, int W, int H,
                               int32_t* out, int conn, hipStream_t stream,
-                              uint32_t* roots) {
-  if (!labels || !out)
-    return fail(SPLIT_HIP_ERR_ARGUMENT, "labels or out is null");
+                              uint32_t* roots, unsigned* flag) {
+  if (!labels || !out || !flag)
+    return fail(SPLIT_HIP_ERR_ARGUMENT, "labels, out or temp is null");
   if (W < 1 || H < 1)
     return fail(SPLIT_HIP_ERR_ARGUMENT, "image is %d x %d", W, H);
   if ((int64_t)W * H > INT_MAX)
@@ -320,24 +336,32 @@ int connected_components_impl(const int32_t* labels, int W, int H,
   if (nty > 65535)
     return fail(SPLIT_HIP_ERR_UNSUPPORTED, "height %d too large", H);
   const dim3 tiles(ntx, nty);
+  const int64_t n = (int64_t)W * H;
+  const dim3 pgrid(grid_for(n, 256, 8));
   if (conn == 8) {
     hipLaunchKernelGGL(ccl_tile<8>, tiles, dim3(kCcThreads), 0, stream,
-                       labels, W, H, out);
+                       labels, W, H, out, flag);
     hipLaunchKernelGGL(ccl_merge<8>, tiles, dim3(3 * kCcTile), 0, stream,
                        labels, W, H, out);
   } else {
     hipLaunchKernelGGL(ccl_tile<4>, tiles, dim3(kCcThreads), 0, stream,
-                       labels, W, H, out);
+                       labels, W, H, out, flag);
     hipLaunchKernelGGL(ccl_merge<4>, tiles, dim3(3 * kCcTile), 0, stream,
                        labels, W, H, out);
   }
   SPLIT_HIP_CHECK_LAUNCH("ccl_tile/ccl_merge");
-  const int64_t n = (int64_t)W * H;
-  hipLaunchKernelGGL(ccl_jump, dim3(grid_for(n, 256, 8)), dim3(256), 0, stream,
-                     out, n, W);
-  hipLaunchKernelGGL(ccl_flatten, dim3(grid_for(n, 256, 8)), dim3(256), 0,
-                     stream, out, n, roots);
-  SPLIT_HIP_CHECK_LAUNCH("ccl_flatten");
+  hipLaunchKernelGGL(ccl_jump, pgrid, dim3(256), 0, stream, out, n, W);
+  hipLaunchKernelGGL(ccl_flatten, pgrid, dim3(256), 0, stream, out, n, roots,
+                     (const unsigned*)nullptr);
+  if (conn == 8)
+    hipLaunchKernelGGL(ccl_check<8>, pgrid, dim3(256), 0, stream, labels, W,
+                       H, out, flag);
+  else
+    hipLaunchKernelGGL(ccl_check<4>, pgrid, dim3(256), 0, stream, labels, W,
+                       H, out, flag);
+  hipLaunchKernelGGL(ccl_flatten, pgrid, dim3(256), 0, stream, out, n, roots,
+                     (const unsigned*)flag);
+  SPLIT_HIP_CHECK_LAUNCH("ccl_flatten/ccl_check");
   return SPLIT_HIP_OK;
 }
 
@@ -423,7 +447,7 @@ int segments_async(const int32_t* labels, int W, int H, int32_t* segments,
   if (!compress) {
     *st_out = nullptr;
     return connected_components_impl(labels, W, H, segments, conn, stream,
-                                     nullptr);
+                                     nullptr, static_cast<unsigned*>(temp));
   }
   CompressLayout L;
   SPLIT_HIP_TRY(compress_layout(temp, temp_bytes, n, &L));
@@ -432,7 +456,7 @@ int segments_async(const int32_t* labels, int W, int H, int32_t* segments,
     return fail(SPLIT_HIP_ERR_TEMP, "temp holds a bitmap of %lld words, %lld"
                 " needed", (long long)L.cap, (long long)words);
   SPLIT_HIP_TRY(connected_components_impl(labels, W, H, segments, conn,
-                                          stream, L.bm));
+                                          stream, L.bm, &L.st->ccl_flag));
   hipLaunchKernelGGL(compress_init_roots, dim3(1), dim3(1), 0, stream, L.st,
                      n);
   SPLIT_HIP_CHECK_LAUNCH("compress_init_roots");
@@ -462,15 +486,23 @@ using namespace splithip;
 extern "C" {
 
 size_t split_hip_connected_components_workspace(int32_t, int32_t) {
-  return 0;
+  return kCcTempBytes;
 }
 
 int split_hip_connected_components(const int32_t* labels, int32_t width,
-                                   int32_t height, void*, size_t,
-                                   int32_t* out, int32_t, int32_t connectivity,
+                                   int32_t height, void* temp,
+                                   size_t temp_bytes, int32_t* out, int32_t,
+                                   int32_t connectivity,
                                    split_hip_stream_t stream) {
+  if (!labels || !out)
+    return fail(SPLIT_HIP_ERR_ARGUMENT, "labels or out is null");
+  if (!temp || temp_bytes < kCcTempBytes)
+    return fail(SPLIT_HIP_ERR_TEMP,
+                "temp holds %zu bytes, connected_components needs %zu",
+                temp ? temp_bytes : (size_t)0, kCcTempBytes);
   return connected_components_impl(labels, width, height, out, connectivity,
-                                   stream, nullptr);
+                                   stream, nullptr,
+                                   static_cast<unsigned*>(temp));
 }
 
 size_t split_hip_compress_labels_workspace(int64_t n, int64_t value_span) {

@@ -10,6 +10,18 @@ constexpr int kCcTile = 32;
 constexpr int kCcTilePx = kCcTile * kCcTile;
 constexpr int kCcThreads = 256;
 
+// Every access to the global forest after the tile pass is a device-scope
+// atomic load or store: the forest is rewritten in place by several
+// successive kernels (merge, jump, flatten, check), each of whose blocks may
+// run on any XCD, and a value must never be served from, or left dirty in,
+// one XCD's cache after a later kernel has replaced it.
+__device__ __forceinline__ int g_load(const int32_t* par, int64_t i) {
+  return __hip_atomic_load(par + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void g_store(int32_t* par, int64_t i, int v) {
+  __hip_atomic_store(par + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------- LDS forest ---
 //
 // Parents only ever grow (a link puts the smaller root under the larger),
@@ -103,20 +115,16 @@ __device__ __forceinline__ void ccl_tile_body(const int32_t* __restrict__ labels
     const int lx = li & (kCcTile - 1), ly = li / kCcTile;
     if (lx >= tw || ly >= th) continue;
     const int r = lds_find(par, li);
-    out[(int64_t)(y0 + ly) * W + x0 + lx] =
-        (y0 + r / kCcTile) * W + x0 + (r & (kCcTile - 1));
+    g_store(out, (int64_t)(y0 + ly) * W + x0 + lx,
+            (y0 + r / kCcTile) * W + x0 + (r & (kCcTile - 1)));
   }
 }
 
 // ------------------------------------------------------- global forest ---
 
-// Plain (cacheable) loads: a stale parent is an older ancestor, so a find
-// may stop at a former root; the atomicMax in g_union then returns the
-// current parent and the union retries from there, so every retry moves to a
-// strictly larger index and the loop ends.
 __device__ __forceinline__ int g_find(const int* par, int i) {
   for (;;) {
-    const int p = par[i];
+    const int p = g_load(par, i);
     if (p == i) return i;
     i = p;
   }
@@ -194,7 +202,7 @@ __device__ __forceinline__ void ccl_merge_body(const int32_t* __restrict__ label
 // replaces a parent by an ancestor, so the forest stays valid under
 // concurrent walks) and then point straight at the root they found.
 __device__ __forceinline__ void ccl_jump_one(int32_t* out, int64_t p, int W) {
-  const int q = out[p];
+  const int q = g_load(out, p);
   if (q == p) return;
   const int py = (int)(p / W), px = (int)(p - (int64_t)py * W);
   const int qy = q / W, qx = q - qy * W;
@@ -202,30 +210,64 @@ __device__ __forceinline__ void ccl_jump_one(int32_t* out, int64_t p, int W) {
     return;  // an in-tile parent: its local root does the walk
   int x = q;
   for (;;) {
-    const int px2 = out[x];
+    const int px2 = g_load(out, x);
     if (px2 == x) break;
-    const int gp = out[px2];
+    const int gp = g_load(out, px2);
     if (gp == px2) {
       x = px2;
       break;
     }
-    out[x] = gp;
+    g_store(out, x, gp);
     x = gp;
   }
-  out[p] = x;
+  g_store(out, p, x);
 }
 
 // Follows the chain of pixel p to its root and writes it back.
 __device__ __forceinline__ int ccl_flatten_one(int32_t* out, int64_t p) {
-  const int r0 = out[p];
+  const int r0 = g_load(out, p);
   int r = r0;
   for (;;) {
-    const int q = out[r];
+    const int q = g_load(out, r);
     if (q == r) break;
     r = q;
   }
-  if (r != r0) out[p] = r;
+  if (r != r0) g_store(out, p, r);
   return r;
+}
+
+// Check of the flattened forest at pixel p: p's value must be a root, and
+// every equal-label neighbour to the right and below (and on the two lower
+// diagonals with 8-connectivity) must carry the same value.  Together these
+// mean each component maps to one root, the largest index of its tree.  A
+// neighbour pair that disagrees is united (the roots are joined as in the
+// merge pass); any failure sets *flag, after which a second flatten pass
+// runs.
+template <int CONN>
+__device__ __forceinline__ void ccl_check_one(const int32_t* __restrict__ labels,
+                                              int W, int H, int32_t* out,
+                                              int64_t p, unsigned* flag) {
+  const int y = (int)(p / W), x = (int)(p - (int64_t)y * W);
+  const int l = labels[p];
+  const int r = g_load(out, p);
+  bool bad = g_load(out, r) != r;
+  auto pair = [&](int64_t q) {
+    if (labels[q] != l) return;
+    const int rq = g_load(out, q);
+    if (rq != r) {
+      g_union(out, r, rq);
+      bad = true;
+    }
+  };
+  if (x + 1 < W) pair(p + 1);
+  if (y + 1 < H) {
+    pair(p + W);
+    if (CONN == 8) {
+      if (x > 0) pair(p + W - 1);
+      if (x + 1 < W) pair(p + W + 1);
+    }
+  }
+  if (bad) atomicOr(flag, 1u);
 }
 
 }  // namespace splithip

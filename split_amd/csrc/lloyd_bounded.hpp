@@ -12,35 +12,57 @@
 
 namespace splithip {
 
-constexpr int kDmSlots = 256;    // drift history entries; t0 is 8 bits
+constexpr int kMaxTileK = 256;  // labels are 8 bits of the per-pixel state
 
 struct TileArgs : Geometry {
   int k;
   float* cbuf;                // two 3k centroid buffers
   long long* totals;          // [4][k]: quantised channel sums, counts
-  float* dm;                  // cumulative max drift, kDmSlots entries
-  uint32_t* pstate;           // per-pixel state (see above)
+  float* fcum;                // per-centroid bound consumption F[j], k
+  uint32_t* pstate;           // per-pixel state (see pack_state)
   const unsigned* maxabs;     // bits of max |v| (non-negative float)
   LloydState* state;
   Mailbox* mbox;
 };
 
-// fp16 bits of s >= 0 rounded down (to the next smaller half when the
-// nearest one is above s; a value past the fp16 range becomes 65504).
-__device__ __forceinline__ unsigned short half_bits_down(float s) {
-  const _Float16 h = (_Float16)s;
-  unsigned short b = __builtin_bit_cast(unsigned short, h);
-  if ((float)h > s && b > 0) b -= 1;
-  return b;
+// Per-pixel state: label | budget, where budget = slack + F[label] at the
+// iteration the pixel was assigned, rounded down, and slack is a lower
+// bound on (distance to the second-nearest centroid - distance to the
+// nearest).  F[j] sums, over the centroid updates so far, the move of
+// centroid j plus the largest move of any other centroid (rounded up).  If
+// pixel p took label a when F[a] was F0, then after further updates its
+// distance to c_a grew by at most the moves of a and its distance to every
+// other centroid shrank by at most the largest other move, so the label is
+// provably unchanged while budget - F[a] > margin (Hamerly's bound, with
+// the history folded into one value per pixel).  The budget keeps the top
+// 24 bits of its float (sign, exponent, 15 mantissa bits); dropping the low
+// bits rounds a non-negative value down.
+// a + b (both >= 0) rounded down; x >= 0 to float rounded up.
+__device__ __forceinline__ float add_down(float a, float b) {
+  const float s = a + b;
+  return (double)s > (double)a + (double)b
+             ? __uint_as_float(__float_as_uint(s) - 1u)
+             : s;
 }
-__device__ __forceinline__ float half_bits_to_float(unsigned short b) {
-  return (float)__builtin_bit_cast(_Float16, b);
+__device__ __forceinline__ float to_float_up(double x) {
+  const float f = (float)x;
+  return (double)f < x ? __uint_as_float(__float_as_uint(f) + 1u) : f;
 }
 
-__device__ __forceinline__ uint32_t pack_state(int label, int t0, float slack) {
-  const unsigned short h = half_bits_down(slack);
-  return (uint32_t)(label & 0xFF) | ((uint32_t)(t0 & 0xFF) << 8) |
-         ((uint32_t)h << 16);
+__device__ __forceinline__ uint32_t pack_state(int label, float budget) {
+  return (__float_as_uint(budget) & 0xFFFFFF00u) | (uint32_t)(label & 0xFF);
+}
+__device__ __forceinline__ float state_budget(uint32_t s) {
+  return __uint_as_float(s & 0xFFFFFF00u);
+}
+
+// F[j] after one more centroid update: + the move of centroid j + the
+// largest move of any other centroid, rounded up.  d2j = squared move of j;
+// m1, m2 = the largest and second-largest squared moves, arg = index of m1.
+__device__ __forceinline__ float bound_step(float f, double d2j, int j,
+                                            double m1, double m2, int arg) {
+  const double other = sqrt(j == arg ? m2 : m1);
+  return to_float_up((double)f + (sqrt(d2j) + other) * (1.0 + 1e-5) + 1e-7);
 }
 
 // Re-assigns up to 256 pixels (4 per lane: idx/v) whose previous labels are
@@ -49,9 +71,9 @@ __device__ __forceinline__ uint32_t pack_state(int label, int t0, float slack) {
 template <int CPL, bool FULL>
 __device__ __forceinline__ void reassign(
     const TileArgs& a, const float (&cx)[CPL], const float (&cy)[CPL],
-    const float (&cz)[CPL], const float (&cn)[CPL], int it, float fscale,
-    int lane, const int64_t (&idx)[4], const bool (&v)[4], bool lvec,
-    const int (&old)[4], long long* slice, IRun& run) {
+    const float (&cz)[CPL], const float (&cn)[CPL], const float* fl,
+    float fscale, int lane, const int64_t (&idx)[4], const bool (&v)[4],
+    bool lvec, const int (&old)[4], long long* slice, IRun& run) {
   const int k = a.k;
   float x[4], y[4], z[4];
   load_colours(a, idx, v, lvec, x, y, z);
@@ -60,7 +82,8 @@ __device__ __forceinline__ void reassign(
   assign_segment<CPL, true>(cx, cy, cz, cn, k, lane, x, y, z, v, lab, slack);
   uint32_t ns[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) ns[i] = pack_state(lab[i], it, slack[i]);
+  for (int i = 0; i < 4; ++i)
+    ns[i] = pack_state(lab[i], v[i] ? add_down(slack[i], fl[v[i] ? lab[i] : 0]) : 0.f);
   if (lvec) {
     *reinterpret_cast<uint4*>(a.pstate + idx[0]) =
         make_uint4(ns[0], ns[1], ns[2], ns[3]);
@@ -123,8 +146,8 @@ template <int CPL, int MODE, bool FULL>
 __device__ __forceinline__ void walk_tiles(
     const TileArgs& a, int64_t s0, int64_t s1, const float (&cx)[CPL],
     const float (&cy)[CPL], const float (&cz)[CPL], const float (&cn)[CPL],
-    int it, float fscale, float margin, float drift_now, const float* dmt,
-    long long* slice, int* qidx, int* qold, int lane) {
+    const float* fl, float fscale, float margin, long long* slice, int* qidx,
+    int* qold, int lane) {
   const int k = a.k;
   IRun run{0, 0, 0, 0, -1, 0};
   int qlen = 0;  // wave-uniform
@@ -142,7 +165,7 @@ __device__ __forceinline__ void walk_tiles(
       idx[i] = qidx[v[i] ? e : 0];
       old[i] = v[i] ? qold[e] : -1;
     }
-    reassign<CPL, false>(a, cx, cy, cz, cn, it, fscale, lane, idx, v, false,
+    reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, false,
                          old, slice, run);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -161,7 +184,7 @@ __device__ __forceinline__ void walk_tiles(
     segment_index<MODE>(a, s, lane, idx, v, lvec);
     if constexpr (FULL) {
       const int old[4] = {-1, -1, -1, -1};
-      reassign<CPL, true>(a, cx, cy, cz, cn, it, fscale, lane, idx, v, lvec,
+      reassign<CPL, true>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, lvec,
                           old, slice, run);
     } else {
       uint32_t ps[4];
@@ -179,15 +202,13 @@ __device__ __forceinline__ void walk_tiles(
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         old[i] = v[i] ? (int)(ps[i] & 0xFFu) : -1;
-        const int t0 = (int)((ps[i] >> 8) & 0xFFu);
-        const float sl = half_bits_to_float((unsigned short)(ps[i] >> 16));
-        fail[i] = v[i] && !(sl - 2.f * (drift_now - dmt[t0]) > margin);
+        fail[i] = v[i] && !(state_budget(ps[i]) - fl[v[i] ? old[i] : 0] > margin);
         fb[i] = ballot(fail[i]);
         nfail += __builtin_popcountll(fb[i]);
       }
       if (nfail == 0) continue;  // every pixel provably unchanged
       if (nfail > kWholeTileFails) {
-        reassign<CPL, false>(a, cx, cy, cz, cn, it, fscale, lane, idx, v, lvec,
+        reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, lvec,
                              old, slice, run);
         continue;
       }

@@ -1,24 +1,21 @@
 // Bounded Lloyd iterations over 16x16 tiles with exact integer sums.
 //
-// Used by cluster_image / segment when k <= 256.  Results equal plain Lloyd:
+// Used by cluster_image / segment when k <= 256 (any iteration count).  Results equal plain Lloyd:
 // every label is the dense float argmin (ties to the lowest index), and the
 // centroids are exact functions of cluster membership.
 //
-// Per-pixel state (4 bytes, in temp): label | t0 << 8 | fp16 slack << 16,
-// where `slack` is a lower bound on (distance to the second-nearest centroid
-// - distance to the nearest one) for the centroid set number t0, rounded
-// down.  If centroid j moves by delta_j between sets, distances move by at
-// most delta_j (triangle inequality), so a pixel provably keeps its label
-// while
-//     slack - 2 * (DM[t] - DM[t0]) > margin,
-// DM[t] being the cumulative sum of max_j delta_j up to set t (Hamerly's
-// bound with the largest drift on both sides).  margin (1e-5 of the data
-// scale) exceeds the float error of the distances, so the float argmin is
-// unchanged too.
+// Per-pixel state (4 bytes, in temp): the label and a "budget" (see
+// pack_state in lloyd_bounded.hpp): a pixel provably keeps its label while
+// budget - F[label] > margin, where F[j] accumulates, over the centroid
+// updates, the move of centroid j plus the largest move of any other
+// centroid (Hamerly's bound; triangle inequality on both distances).  margin
+// (1e-5 of the data scale) exceeds the float error of the distances, so the
+// float argmin is unchanged too.
 //
 // Iteration 1 assigns every tile.  Later iterations read the state, skip
 // tiles whose every pixel passes the test without touching their colours,
-// and re-assign the others whole (writing fresh state).
+// re-assign tiles with many failing pixels whole, and queue the other
+// failing pixels for re-assignment in batches of 256 (writing fresh state).
 //
 // Sums: channels are quantised once, q = trunc(v * 2^S) with |q| <= 2^22
 // (S from the largest |v| of the image), and summed as integers into int64
@@ -36,14 +33,13 @@
 
 namespace splithip {
 
-// Last block: totals -> centroids, shift, drift history, stop decision.
+// Last block: totals -> centroids, shift, bound consumption, stop decision.
 __device__ void tiles_finalize(const TileArgs& a, const float* cent,
                                float* cent_out, float fscale, double* scratch) {
   const int k = a.k;
   const int tid = threadIdx.x, nt = blockDim.x;
   double* d2 = scratch;             // k entries: |new - old|^2
   double* part = scratch + k;       // kWave partial sums
-  double* pmax = part + kWave;      // kWave partial maxima
   const double inv = 1.0 / (double)fscale;
   for (int j = tid; j < k; j += nt) {
     const long long cnt = __hip_atomic_load(&a.totals[3 * k + j],
@@ -64,26 +60,41 @@ __device__ void tiles_finalize(const TileArgs& a, const float* cent,
   }
   __syncthreads();
   if (tid < kWave) {
-    double s = 0.0, m = 0.0;
-    for (int j = tid; j < k; j += kWave) {
-      s += d2[j];
-      m = fmax(m, d2[j]);
-    }
+    double s = 0.0;
+    for (int j = tid; j < k; j += kWave) s += d2[j];
     part[tid] = s;
-    pmax[tid] = m;
   }
   __syncthreads();
+  __shared__ double top[2];
+  __shared__ int top_arg;
+  __shared__ float shift_sh;
   if (tid == 0) {
-    double s = 0.0, m = 0.0;
-    for (int j = 0; j < kWave; ++j) {
-      s += part[j];
-      m = fmax(m, pmax[j]);
+    double s = 0.0;
+    for (int j = 0; j < kWave; ++j) s += part[j];
+    shift_sh = (float)sqrt(s);
+    double m1 = 0.0, m2 = 0.0;
+    int arg = -1;
+    for (int j = 0; j < k; ++j) {
+      const double d = d2[j];
+      if (d > m1) {
+        m2 = m1;
+        m1 = d;
+        arg = j;
+      } else if (d > m2) {
+        m2 = d;
+      }
     }
+    top[0] = m1;
+    top[1] = m2;
+    top_arg = arg;
+  }
+  __syncthreads();
+  for (int j = tid; j < k; j += nt)
+    a.fcum[j] = bound_step(a.fcum[j], d2[j], j, top[0], top[1], top_arg);
+  if (tid == 0) {
     LloydState* st = a.state;
-    const float shift = (float)sqrt(s);
+    const float shift = shift_sh;
     const int it = st->iter + 1;
-    // rounded up: the skip test must never under-estimate the drift
-    a.dm[it] = (float)((double)a.dm[it - 1] + sqrt(m) * (1.0 + 1e-5) + 1e-7);
     const int done = !(shift > st->threshold) || it >= st->max_iter;
     st->iter = it;
     st->shift = shift;
@@ -110,19 +121,17 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
   const LloydState* st = a.state;
   if (st->done) return;
   const int cur = st->cur;
-  const int it = st->iter;  // index of the centroid set assigned against
   const float* cent = a.cbuf + (size_t)cur * 3 * k;
   float* cent_out = a.cbuf + (size_t)(1 - cur) * 3 * k;
 
   const int V = 4 * k;
   long long* slices = lds64;  // nwaves x V
-  float* dmt = reinterpret_cast<float*>(
-      lds64 + std::max(nwaves * V, k + 2 * kWave));
-  int* qidx = reinterpret_cast<int*>(dmt + kDmSlots) + wave * 2 * kQueueCap;
+  float* fl = reinterpret_cast<float*>(
+      lds64 + std::max(nwaves * V, k + 2 * kWave));  // F[0..k)
+  int* qidx = reinterpret_cast<int*>(fl + kMaxTileK) + wave * 2 * kQueueCap;
   int* qold = qidx + kQueueCap;
   for (int i = threadIdx.x; i < nwaves * V; i += blockDim.x) slices[i] = 0;
-  if constexpr (!FULL)
-    for (int i = threadIdx.x; i <= it; i += blockDim.x) dmt[i] = a.dm[i];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) fl[i] = a.fcum[i];
 
   float cx[CPL], cy[CPL], cz[CPL], cn[CPL];
 #pragma unroll
@@ -138,7 +147,6 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
   const float fscale = scale_of(*a.maxabs);
   const float margin = 1e-5f * (1.f + 4.f * maxabs);
   __syncthreads();
-  const float drift_now = FULL ? 0.f : dmt[it];
   long long* slice = slices + (size_t)wave * V;
 
   const int64_t gw = (int64_t)blockIdx.x * nwaves + wave;
@@ -146,8 +154,8 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
   const int64_t s0 = a.nseg * gw / W;
   const int64_t s1 = a.nseg * (gw + 1) / W;
 
-  walk_tiles<CPL, MODE, FULL>(a, s0, s1, cx, cy, cz, cn, it, fscale, margin,
-                              drift_now, dmt, slice, qidx, qold, lane);
+  walk_tiles<CPL, MODE, FULL>(a, s0, s1, cx, cy, cz, cn, fl, fscale, margin,
+                              slice, qidx, qold, lane);
   __syncthreads();
   for (int vv = threadIdx.x; vv < V; vv += blockDim.x) {
     long long tot = 0;
@@ -169,9 +177,10 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
 
 __global__ void tiles_init(LloydState* st, float* cbuf, const float* cent,
                            int k, int max_iter, float threshold,
-                           long long* totals, float* dm, unsigned* maxabs) {
+                           long long* totals, float* fcum, unsigned* maxabs) {
   for (int i = threadIdx.x; i < 3 * k; i += blockDim.x) cbuf[i] = cent[i];
   for (int i = threadIdx.x; i < 4 * k; i += blockDim.x) totals[i] = 0;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) fcum[i] = 0.f;
   if (threadIdx.x == 0) {
     st->arrive = 0u;
     st->cur = 0;
@@ -181,7 +190,6 @@ __global__ void tiles_init(LloydState* st, float* cbuf, const float* cent,
     st->threshold = threshold;
     st->shift = 0.f;
     st->reserved = 0;
-    dm[0] = 0.f;
     *maxabs = 0u;
   }
 }
@@ -240,7 +248,7 @@ struct TileLayout {
   LloydState* st;
   float* cbuf;
   long long* totals;
-  float* dm;
+  float* fcum;
   unsigned* maxabs;
   uint32_t* pstate;
   size_t bytes;
@@ -258,7 +266,7 @@ static TileLayout tiles_layout(void* temp, int64_t n, int k) {
   L.st = reinterpret_cast<LloydState*>(take(sizeof(LloydState)));
   L.cbuf = reinterpret_cast<float*>(take(6 * (size_t)k * sizeof(float)));
   L.totals = reinterpret_cast<long long*>(take(4 * (size_t)k * 8));
-  L.dm = reinterpret_cast<float*>(take(kDmSlots * sizeof(float)));
+  L.fcum = reinterpret_cast<float*>(take((size_t)k * sizeof(float)));
   L.maxabs = reinterpret_cast<unsigned*>(take(sizeof(unsigned)));
   L.pstate = reinterpret_cast<uint32_t*>(take((size_t)n * sizeof(uint32_t)));
   L.bytes = o;
@@ -279,7 +287,8 @@ size_t tiles_workspace(int64_t n, int k) {
 }
 
 bool tiles_supported(int width, int k, int max_iter) {
-  return width > 0 && k >= 1 && k <= 256 && max_iter <= kDmSlots - 2;
+  (void)max_iter;
+  return width > 0 && k >= 1 && k <= kMaxTileK;
 }
 
 template <bool FULL, int MODE>
@@ -318,7 +327,7 @@ int cluster_tiles_async(const float* pts, int width, int height,
   res->host->shift = 0.f;
 
   hipLaunchKernelGGL(tiles_init, dim3(1), dim3(256), 0, stream, L.st, L.cbuf,
-                     centroids, k, max_iter, threshold, L.totals, L.dm,
+                     centroids, k, max_iter, threshold, L.totals, L.fcum,
                      L.maxabs);
   const int vec = ((uintptr_t)pts & 15u) == 0;
   hipLaunchKernelGGL(tiles_absmax, dim3(8 * device_cu_count()), dim3(256), 0,
@@ -335,7 +344,7 @@ int cluster_tiles_async(const float* pts, int width, int height,
   a.k = k;
   a.cbuf = L.cbuf;
   a.totals = L.totals;
-  a.dm = L.dm;
+  a.fcum = L.fcum;
   a.pstate = L.pstate;
   a.maxabs = L.maxabs;
   a.state = L.st;
@@ -346,7 +355,7 @@ int cluster_tiles_async(const float* pts, int width, int height,
   const int64_t want = (a.nseg + nwaves - 1) / nwaves;
   const size_t lds =
       std::max<size_t>((size_t)nwaves * 4 * k, (size_t)k + 2 * kWave) * 8 +
-      kDmSlots * sizeof(float) +
+      kMaxTileK * sizeof(float) +
       (size_t)nwaves * 2 * kQueueCap * sizeof(int);
   // resident blocks per CU of the bounded kernel (its register use is the
   // larger of the two), so that the grid is one wave of blocks

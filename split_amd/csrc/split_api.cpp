@@ -184,7 +184,10 @@ void connected_components(const_view2d<int> di_labels,
 {
   check(split_hip_connected_components(
             di_labels.values.data(), (int32_t)di_labels.num_cols,
-            (int32_t)di_labels.num_rows, dio_temp.get(), 0, do_labels.data(),
+            (int32_t)di_labels.num_rows, dio_temp.get(),
+            split_hip_connected_components_workspace(
+                (int32_t)di_labels.num_cols, (int32_t)di_labels.num_rows),
+            do_labels.data(),
             i_max_iterations, i_connectivity, nullptr),
         "connected_components");
   sync("connected_components");

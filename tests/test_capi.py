@@ -38,7 +38,7 @@ def test_host_only_entry_points_without_gpu():
     assert lib.split_hip_version().decode() == "0.1.0"
     assert lib.split_hip_status_string(2).decode() == "temp buffer too small"
     assert lib.split_hip_label_points_workspace(1000, 8) == 0
-    assert lib.split_hip_connected_components_workspace(64, 64) == 0
+    assert lib.split_hip_connected_components_workspace(64, 64) == 256
     ws = lib.split_hip_cluster_workspace(4096 * 4096, 32, 3)
     n = 4096 * 4096
     assert 4 * n <= ws < 4 * n + (1 << 20)  # per-pixel Lloyd state + small tables

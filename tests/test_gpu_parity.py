@@ -386,7 +386,30 @@ def test_cc_spiral_stress_2048(torch_cuda, conn):
     lab = W.make_cc_stress(2048, 2048)
     got = run_cc(torch, lab, conn)
     want, ncomp = oracle.cc_uf(lab, conn)
-    assert np.array_equal(got, want)
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, (f"{bad.size} pixels differ; first {bad[:5].tolist()} "
+                           f"got {got[bad[:5]].tolist()} want {want[bad[:5]].tolist()}")
+
+
+def test_cc_spiral_repeated_runs_agree(torch_cuda):
+    """Regression: the global union-find must not lose a union to a stale
+    cached parent; every run of the long-chain spiral image is exact."""
+    torch = torch_cuda
+    lab = W.make_cc_stress(2048, 2048)
+    L = dev(torch, lab)
+    out = torch.empty(lab.size, dtype=torch.int32, device="cuda")
+    temp = torch.zeros(256, dtype=torch.uint8, device="cuda")
+    repairs = 0
+    for conn in (4, 8):
+        want, _ = oracle.cc_uf(lab, conn)
+        for rep in range(4):
+            say(f"cc spiral 2048 conn={conn} run {rep}")
+            sa.connected_components(L, out, connectivity=conn, temp=temp)
+            got = host(out)
+            repairs += int(temp[:4].view(torch.int32).item() != 0)
+            bad = np.flatnonzero(got != want)
+            assert bad.size == 0, (conn, rep, bad.size, bad[:5].tolist())
+    say(f"   check pass repaired {repairs} of 8 runs")
 
 
 # ------------------------------------------------------------- compress ---
