@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--cpu-crop", type=int, default=1024)
+    ap.add_argument("--no-warm-start", action="store_true",
+                    help="skip the learned warm-start comparison")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes behind roofline.traffic")
     ap.add_argument("--profile-driver", action="store_true",
@@ -488,6 +490,26 @@ def main():
                                                     round(t["write_scale"], 4)]
                 log(f"bench: traffic {t['read'] / 1e6:.1f} MB read + {t['write'] / 1e6:.1f} MB "
                     f"written per launch")
+
+    if not args.no_warm_start:
+        # The learned warm start against the plain method on its evaluation
+        # set (256 config-2-style 1024x1024 images, k=16): plain separate
+        # segment() calls, plain batched, warm-started batched.
+        from split_amd import warmstart as WS
+        log("bench: warm-start evaluation (256 x 1024^2, k=16)")
+        ev = WS.evaluate(WS.WarmStartNet.load(), count=256, side=1024, separate=True,
+                         log=log)["summary"]
+        out["warm_start"] = {
+            "workload": "256 x 1024x1024 shaded Voronoi, k=16, shift<1e-4 or 100 iters",
+            "plain_separate_mpix_s": round(ev["separate_plain_mpix_s"], 1),
+            "plain_batched_mpix_s": round(ev["plain_mpix_s"], 1),
+            "warm_batched_mpix_s": round(ev["warm_mpix_s"], 1),
+            "speedup_warm_batched_vs_plain_separate":
+                round(ev["speedup_batched_warm_vs_separate_plain"], 3),
+            "plain_iters_mean": round(ev["plain_iters_mean"], 2),
+            "warm_iters_mean": round(ev["warm_iters_mean"], 2),
+            "inertia_ratio_warm_vs_plain": round(ev["inertia_ratio_total"], 4),
+        }
 
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")

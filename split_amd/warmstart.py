@@ -286,12 +286,14 @@ def eval_specs(count: int = 256, side: int = 1024, seed: int = 0xE7A1_0004):
 
 
 def evaluate(net: "WarmStartNet", count: int = 256, side: int = 1024, reps: int = 1,
-             log=print) -> dict:
+             separate: bool = False, log=print) -> dict:
     """Plain (seeded-sample init) against warm-started (network init) batched
     segmentation of the evaluation images: Lloyd iterations to shift < 1e-4
-    (at most 100), final inertia, and time per run (init + segmentation)."""
+    (at most 100), final inertia, and time per run (init + segmentation).
+    With `separate`, also the plain method as one segment() call per image
+    (the unbatched GPU path), whose results equal the batched plain run."""
     torch = _torch()
-    from . import gen_images, inertia
+    from . import gen_images, inertia, segment, segment_workspace
     specs = eval_specs(count, side)
     n = side * side
     planes = torch.empty((count, 3, n), dtype=torch.float32, device="cuda")
@@ -335,6 +337,29 @@ def evaluate(net: "WarmStartNet", count: int = 256, side: int = 1024, reps: int 
             log(f"eval {name}: {best * 1e3:.1f} ms, iterations mean {its.mean():.2f} "
                 f"(median {np.median(its):.0f}, hit cap {(its >= 100).sum()}), "
                 f"inertia sum {ine.sum():.2f}")
+    if separate:
+        stemp = torch.empty(segment_workspace(side, side, K), dtype=torch.uint8,
+                            device="cuda")
+
+        def run_separate():
+            res = []
+            for i, (_, init_seed) in enumerate(specs):
+                uniform_random_initialize(planes[i], cent[i], init_seed)
+                res.append(segment(planes[i], side, side, cent[i], labels[i], segs[i],
+                                   max_iter=100, threshold=1e-4, connectivity=8,
+                                   temp=stemp))
+            return res
+
+        run_separate()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = run_separate()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        same = all(a.iterations == b for a, b in zip(res, out["plain"]["iterations"]))
+        out["separate"] = {"seconds": dt, "same_iterations": same}
+        if log:
+            log(f"eval separate plain calls: {dt * 1e3:.1f} ms (same iterations: {same})")
     ratio = out["warm"]["inertia"] / out["plain"]["inertia"]
     out["summary"] = {
         "images": count, "side": side, "pixels": count * n,
@@ -350,6 +375,14 @@ def evaluate(net: "WarmStartNet", count: int = 256, side: int = 1024, reps: int 
         "inertia_ratio_max": float(ratio.max()),
         "images_warm_worse_0p5pct": int((ratio > 1.005).sum()),
     }
+    if separate:
+        sep = out["separate"]["seconds"]
+        out["summary"].update({
+            "separate_plain_ms": sep * 1e3,
+            "separate_plain_mpix_s": count * n / sep / 1e6,
+            "speedup_batched_warm_vs_separate_plain": sep / out["warm"]["seconds"],
+            "separate_same_iterations": out["separate"]["same_iterations"],
+        })
     return out
 
 
@@ -382,7 +415,7 @@ def main(argv=None):
         log(f"saved {args.out}")
     else:
         net = WarmStartNet.load(args.weights)
-        print(json.dumps(evaluate(net, args.images, log=log)["summary"]))
+        print(json.dumps(evaluate(net, args.images, separate=True, log=log)["summary"]))
 
 
 if __name__ == "__main__":
