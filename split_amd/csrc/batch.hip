@@ -387,11 +387,13 @@ __global__ __launch_bounds__(512) void batch_step_bounded(BatchArgs a, Group g) 
                      ((((uintptr_t)ta.pstate) & 15u) == 0);
     const int64_t t0 = s - m.tile0, t1 = end - m.tile0;
     if (vec)
-      walk_tiles<1, kTileVec, FIRST>(ta, t0, t1, cx, cy, cz, cn, w.fl, fscale,
-                                     margin, w.slice, w.qidx, w.qold, lane);
+      walk_tiles<1, kTileVec, FIRST>(ta, TileRange{t0, t1}, cx, cy, cz,
+                                     cn, w.fl, fscale, margin, w.slice,
+                                     w.qidx, w.qold, lane);
     else
-      walk_tiles<1, kTile, FIRST>(ta, t0, t1, cx, cy, cz, cn, w.fl, fscale,
-                                  margin, w.slice, w.qidx, w.qold, lane);
+      walk_tiles<1, kTile, FIRST>(ta, TileRange{t0, t1}, cx, cy, cz, cn,
+                                  w.fl, fscale, margin, w.slice, w.qidx,
+                                  w.qold, lane);
     batch_finish_segment(a, m, (int)(end - s), w, lane);
     s = end;
   }

@@ -130,7 +130,7 @@ struct LloydState {
   int max_iter;
   float threshold;
   float shift;      // Frobenius norm of the last centroid update
-  int reserved;
+  unsigned next;    // tile chunks handed out in the current launch
 };
 
 // Host-mapped mailbox the last block of a step launch writes, so the host

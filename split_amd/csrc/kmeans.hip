@@ -349,7 +349,7 @@ __global__ void lloyd_init(LloydState* st, float* cbuf, const float* cent,
     st->max_iter = max_iter;
     st->threshold = threshold;
     st->shift = 0.f;
-    st->reserved = 0;
+    st->next = 0u;
   }
 }
 

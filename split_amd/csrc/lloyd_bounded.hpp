@@ -18,11 +18,16 @@ struct TileArgs : Geometry {
   int k;
   float* cbuf;                // two 3k centroid buffers
   long long* totals;          // [4][k]: quantised channel sums, counts
+  long long* partials;        // [blocks][4k]: each block's share of a step
   float* fcum;                // per-centroid bound consumption F[j], k
   uint32_t* pstate;           // per-pixel state (see pack_state)
   const unsigned* maxabs;     // bits of max |v| (non-negative float)
   LloydState* state;
   Mailbox* mbox;
+  // spread order of the bounded iterations (see SpreadUnits): units of two
+  // tiles, units per tile row, multiplier coprime to nunits
+  int64_t nunits, unit_mul;
+  int units_per_row;
 };
 
 // Per-pixel state: label | budget, where budget = slack + F[label] at the
@@ -133,18 +138,61 @@ __device__ __forceinline__ void reassign(
   }
 }
 
+// Tile source for walk_tiles: the tiles [s, end) in order.
+struct TileRange {
+  int64_t s, end;
+  __device__ bool operator()(int64_t& t) {
+    if (s >= end) return false;
+    t = s++;
+    return true;
+  }
+};
+
+// Tile source for walk_tiles shared by the waves of one block: units of two
+// horizontally adjacent tiles (so a wave reads whole 128-byte rows of the
+// state), taken one at a time from a counter in LDS.  The block owns the
+// units p(r) for ranks r in [r0, r1), p(r) = r * mul mod nunits with mul
+// coprime to nunits, so that consecutive ranks land far apart in the image:
+// tiles that fail the skip test come in large patches, and spreading every
+// block's units over the whole image evens out the work per block.
+struct SpreadUnits {
+  int64_t r0, r1, nunits, mul;
+  int units_per_row, tiles_x;
+  unsigned* counter;  // LDS, zero before the first call
+  int lane;
+  int64_t pending;    // second tile of the current unit, or -1
+  __device__ bool operator()(int64_t& t) {
+    if (pending >= 0) {
+      t = pending;
+      pending = -1;
+      return true;
+    }
+    unsigned i = 0u;
+    if (lane == 0) i = atomicAdd(counter, 1u);
+    const int64_t r = r0 + (int64_t)(unsigned)readfirstlane_i((int)i);
+    if (r >= r1) return false;
+    const int64_t u = r * mul % nunits;
+    const int64_t row = u / units_per_row;
+    const int c = (int)(u - row * units_per_row);
+    t = row * tiles_x + 2 * c;
+    if (2 * c + 1 < tiles_x) pending = t + 1;
+    return true;
+  }
+};
+
 constexpr int kQueueCap = 512;      // per-wave queue of failing pixels
 constexpr int kWholeTileFails = 96;  // more failing pixels: re-assign the tile
 
-// Processes tiles [s0, s1) of one image for one wave: FULL re-assigns every
-// pixel (run accumulation), otherwise tiles whose pixels all pass the skip
-// test are left alone, tiles with many failing pixels are re-assigned whole
-// and the other failing pixels are queued and re-assigned in batches of 256.
-// On return every change is in `slice` (the queue is drained, the run
-// flushed).
-template <int CPL, int MODE, bool FULL>
+// Processes, for one wave, the tiles that next(t) hands out until it
+// returns false: FULL re-assigns every pixel (run accumulation), otherwise
+// tiles whose pixels all pass the skip test are left alone, tiles with many
+// failing pixels are re-assigned whole and the other failing pixels are
+// queued and re-assigned in batches of 256.  The state of the next tile is
+// loaded while the current one is tested.  On return every change is in
+// `slice` (the queue is drained, the run flushed).
+template <int CPL, int MODE, bool FULL, class NextTile>
 __device__ __forceinline__ void walk_tiles(
-    const TileArgs& a, int64_t s0, int64_t s1, const float (&cx)[CPL],
+    const TileArgs& a, NextTile next, const float (&cx)[CPL],
     const float (&cy)[CPL], const float (&cz)[CPL], const float (&cn)[CPL],
     const float* fl, float fscale, float margin, long long* slice, int* qidx,
     int* qold, int lane) {
@@ -178,23 +226,33 @@ __device__ __forceinline__ void walk_tiles(
     qlen -= c;
   };
 
-  for (int64_t s = s0; s < s1; ++s) {
-    int64_t idx[4];
-    bool v[4], lvec;
-    segment_index<MODE>(a, s, lane, idx, v, lvec);
-    if constexpr (FULL) {
+  if constexpr (FULL) {
+    int64_t s;
+    while (next(s)) {
+      int64_t idx[4];
+      bool v[4], lvec;
+      segment_index<MODE>(a, s, lane, idx, v, lvec);
       const int old[4] = {-1, -1, -1, -1};
       reassign<CPL, true>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, lvec,
                           old, slice, run);
-    } else {
-      uint32_t ps[4];
-      if (lvec) {
+    }
+  } else {
+    auto load_state = [&](int64_t s, bool have, uint32_t (&ps)[4]) {
+      int64_t idx[4];
+      bool v[4], lvec;
+      segment_index<MODE>(a, have ? s : 0, lane, idx, v, lvec);
+      if (have && lvec) {
         const uint4 q = *reinterpret_cast<const uint4*>(a.pstate + idx[0]);
         ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
       } else {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) ps[i] = v[i] ? a.pstate[idx[i]] : 0u;
+        for (int i = 0; i < 4; ++i) ps[i] = have && v[i] ? a.pstate[idx[i]] : 0u;
       }
+    };
+    auto visit = [&](int64_t s, const uint32_t (&ps)[4]) {
+      int64_t idx[4];
+      bool v[4], lvec;
+      segment_index<MODE>(a, s, lane, idx, v, lvec);
       int old[4];
       bool fail[4];
       uint64_t fb[4];
@@ -202,15 +260,16 @@ __device__ __forceinline__ void walk_tiles(
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         old[i] = v[i] ? (int)(ps[i] & 0xFFu) : -1;
-        fail[i] = v[i] && !(state_budget(ps[i]) - fl[v[i] ? old[i] : 0] > margin);
+        fail[i] = v[i] &&
+                  !(state_budget(ps[i]) - fl[v[i] ? old[i] : 0] > margin);
         fb[i] = ballot(fail[i]);
         nfail += __builtin_popcountll(fb[i]);
       }
-      if (nfail == 0) continue;  // every pixel provably unchanged
+      if (nfail == 0) return;  // every pixel provably unchanged
       if (nfail > kWholeTileFails) {
-        reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, lvec,
-                             old, slice, run);
-        continue;
+        reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v,
+                             lvec, old, slice, run);
+        return;
       }
       // queue the failing pixels (in lane order within each slot)
 #pragma unroll
@@ -225,6 +284,21 @@ __device__ __forceinline__ void walk_tiles(
         qlen += __builtin_popcountll(fb[i]);
       }
       if (qlen >= 256) drain();
+    };
+    int64_t s = 0;
+    bool have = next(s);
+    uint32_t ps[4];
+    load_state(s, have, ps);
+    while (have) {
+      int64_t s2 = 0;
+      const bool have2 = next(s2);
+      uint32_t ps2[4];
+      load_state(s2, have2, ps2);
+      visit(s, ps);
+      s = s2;
+      have = have2;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ps[i] = ps2[i];
     }
   }
   if constexpr (!FULL) {

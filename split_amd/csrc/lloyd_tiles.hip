@@ -33,24 +33,22 @@
 
 namespace splithip {
 
-// Last block: totals -> centroids, shift, bound consumption, stop decision.
-__device__ void tiles_finalize(const TileArgs& a, const float* cent,
-                               float* cent_out, float fscale, double* scratch) {
+// totals (tot, [4][k]) -> centroids, shift, bound consumption, stop
+// decision; one block.
+__device__ void tiles_finalize(const TileArgs& a, const long long* tot,
+                               const float* cent, float* cent_out,
+                               float fscale, double* scratch) {
   const int k = a.k;
   const int tid = threadIdx.x, nt = blockDim.x;
   double* d2 = scratch;             // k entries: |new - old|^2
   double* part = scratch + k;       // kWave partial sums
   const double inv = 1.0 / (double)fscale;
   for (int j = tid; j < k; j += nt) {
-    const long long cnt = __hip_atomic_load(&a.totals[3 * k + j],
-                                            __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT);
+    const long long cnt = tot[3 * k + j];
     double s = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const long long sum = __hip_atomic_load(&a.totals[c * k + j],
-                                              __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+      const long long sum = tot[c * k + j];
       const float nc = cnt > 0 ? (float)((double)sum * inv / (double)cnt) : 0.f;
       const double d = (double)nc - (double)cent[c * k + j];
       s += d * d;
@@ -107,7 +105,6 @@ __device__ void tiles_finalize(const TileArgs& a, const float* cent,
       mb->done = done;
       __threadfence_system();
     }
-    st->arrive = 0u;
   }
 }
 
@@ -120,18 +117,17 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
   const int k = a.k;
   const LloydState* st = a.state;
   if (st->done) return;
-  const int cur = st->cur;
-  const float* cent = a.cbuf + (size_t)cur * 3 * k;
-  float* cent_out = a.cbuf + (size_t)(1 - cur) * 3 * k;
+  const float* cent = a.cbuf + (size_t)st->cur * 3 * k;
 
   const int V = 4 * k;
-  long long* slices = lds64;  // nwaves x V
-  float* fl = reinterpret_cast<float*>(
-      lds64 + std::max(nwaves * V, k + 2 * kWave));  // F[0..k)
+  long long* slices = lds64;                                   // nwaves x V
+  float* fl = reinterpret_cast<float*>(lds64 + nwaves * V);    // F[0..k)
   int* qidx = reinterpret_cast<int*>(fl + kMaxTileK) + wave * 2 * kQueueCap;
   int* qold = qidx + kQueueCap;
   for (int i = threadIdx.x; i < nwaves * V; i += blockDim.x) slices[i] = 0;
   for (int i = threadIdx.x; i < k; i += blockDim.x) fl[i] = a.fcum[i];
+  __shared__ unsigned unit_next;
+  if (threadIdx.x == 0) unit_next = 0u;
 
   float cx[CPL], cy[CPL], cz[CPL], cn[CPL];
 #pragma unroll
@@ -149,30 +145,67 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
   __syncthreads();
   long long* slice = slices + (size_t)wave * V;
 
-  const int64_t gw = (int64_t)blockIdx.x * nwaves + wave;
-  const int64_t W = (int64_t)gridDim.x * nwaves;
-  const int64_t s0 = a.nseg * gw / W;
-  const int64_t s1 = a.nseg * (gw + 1) / W;
-
-  walk_tiles<CPL, MODE, FULL>(a, s0, s1, cx, cy, cz, cn, fl, fscale, margin,
-                              slice, qidx, qold, lane);
+  if constexpr (FULL) {
+    // every tile costs the same: contiguous tiles per wave, so that runs of
+    // one label accumulate in registers
+    const int64_t gw = (int64_t)blockIdx.x * nwaves + wave;
+    const int64_t W = (int64_t)gridDim.x * nwaves;
+    walk_tiles<CPL, MODE, FULL>(
+        a, TileRange{a.nseg * gw / W, a.nseg * (gw + 1) / W}, cx, cy, cz, cn,
+        fl, fscale, margin, slice, qidx, qold, lane);
+  } else {
+    const int64_t G = gridDim.x;
+    walk_tiles<CPL, MODE, FULL>(
+        a,
+        SpreadUnits{a.nunits * blockIdx.x / G, a.nunits * (blockIdx.x + 1) / G,
+                    a.nunits, a.unit_mul, a.units_per_row, a.tiles_x,
+                    &unit_next, lane, -1},
+        cx, cy, cz, cn, fl, fscale, margin, slice, qidx, qold, lane);
+  }
+  // the block's share of the step, summed over its waves; tiles_reduce adds
+  // the shares of all blocks into the totals after this launch ends
   __syncthreads();
+  long long* part = a.partials + (size_t)blockIdx.x * V;
   for (int vv = threadIdx.x; vv < V; vv += blockDim.x) {
     long long tot = 0;
     for (int w = 0; w < nwaves; ++w) tot += slices[(size_t)w * V + vv];
-    if (tot != 0)
-      atomicAdd(reinterpret_cast<unsigned long long*>(&a.totals[vv]),
-                (unsigned long long)tot);
+    part[vv] = tot;
   }
-  __threadfence();
-  __shared__ unsigned ticket;
+}
+
+// One block after each step launch: totals += the shares of the step's
+// `blocks` blocks (fixed order; integer sums), then tiles_finalize.
+constexpr int kReduceThreads = 1024;
+
+__global__ __launch_bounds__(kReduceThreads) void tiles_reduce(TileArgs a,
+                                                                int blocks) {
+  __shared__ long long red[kReduceThreads];
+  __shared__ long long tot[4 * kMaxTileK];
+  __shared__ double scratch[kMaxTileK + kWave];
+  LloydState* st = a.state;
+  if (st->done) return;
+  const int k = a.k, V = 4 * k;
+  const int tid = threadIdx.x;
+  static_assert(4 * kMaxTileK <= kReduceThreads, "one pass over the entries");
+  // thread t sums entry t % V over blocks t / V, t / V + per, ...
+  const int per = kReduceThreads / V;
+  const int v = tid % V, q = tid / V;
+  long long sum = 0;
+  if (q < per)
+    for (int g = q; g < blocks; g += per) sum += a.partials[(size_t)g * V + v];
+  red[tid] = sum;
   __syncthreads();
-  if (threadIdx.x == 0) ticket = atomicAdd(&a.state->arrive, 1u);
+  if (tid < V) {
+    long long t = a.totals[tid];
+    for (int r = 0; r < per; ++r) t += red[r * V + tid];
+    a.totals[tid] = t;
+    tot[tid] = t;
+  }
   __syncthreads();
-  if (ticket != gridDim.x - 1) return;
-  __threadfence();
-  tiles_finalize(a, cent, cent_out, fscale,
-                 reinterpret_cast<double*>(lds64));
+  const int cur = st->cur;
+  const float* cent = a.cbuf + (size_t)cur * 3 * k;
+  float* cent_out = a.cbuf + (size_t)(1 - cur) * 3 * k;
+  tiles_finalize(a, tot, cent, cent_out, scale_of(*a.maxabs), scratch);
 }
 
 __global__ void tiles_init(LloydState* st, float* cbuf, const float* cent,
@@ -182,14 +215,13 @@ __global__ void tiles_init(LloydState* st, float* cbuf, const float* cent,
   for (int i = threadIdx.x; i < 4 * k; i += blockDim.x) totals[i] = 0;
   for (int i = threadIdx.x; i < k; i += blockDim.x) fcum[i] = 0.f;
   if (threadIdx.x == 0) {
-    st->arrive = 0u;
     st->cur = 0;
     st->iter = 0;
     st->done = max_iter <= 0;
     st->max_iter = max_iter;
     st->threshold = threshold;
     st->shift = 0.f;
-    st->reserved = 0;
+    st->next = 0u;
     *maxabs = 0u;
   }
 }
@@ -242,12 +274,19 @@ __global__ void tiles_finish(const LloydState* st, const float* cbuf,
 
 // ------------------------------------------------------------ launchers ---
 
-static int tiles_threads(int k) { return k <= 64 ? 512 : 256; }
+// One block per CU where the LDS allows (k <= 128: up to 129 KB): the
+// waves of a CU then share one pool of tiles in the bounded iterations and
+// finish together.
+static int tiles_threads(int k) { return k <= 128 ? 1024 : 256; }
+// Step launches use at most this many blocks (partials in temp are sized
+// for it).
+static int tiles_max_blocks(int k) { return k <= 128 ? 512 : 1024; }
 
 struct TileLayout {
   LloydState* st;
   float* cbuf;
   long long* totals;
+  long long* partials;
   float* fcum;
   unsigned* maxabs;
   uint32_t* pstate;
@@ -266,11 +305,30 @@ static TileLayout tiles_layout(void* temp, int64_t n, int k) {
   L.st = reinterpret_cast<LloydState*>(take(sizeof(LloydState)));
   L.cbuf = reinterpret_cast<float*>(take(6 * (size_t)k * sizeof(float)));
   L.totals = reinterpret_cast<long long*>(take(4 * (size_t)k * 8));
+  L.partials = reinterpret_cast<long long*>(
+      take((size_t)tiles_max_blocks(k) * 4 * k * 8));
   L.fcum = reinterpret_cast<float*>(take((size_t)k * sizeof(float)));
   L.maxabs = reinterpret_cast<unsigned*>(take(sizeof(unsigned)));
   L.pstate = reinterpret_cast<uint32_t*>(take((size_t)n * sizeof(uint32_t)));
   L.bytes = o;
   return L;
+}
+
+// A multiplier m coprime to n near n * (golden ratio - 1): r -> r * m mod n
+// is a permutation of [0, n) that sends consecutive r far apart.
+static int64_t spread_multiplier(int64_t n) {
+  if (n <= 2) return 1;
+  int64_t m = std::max<int64_t>(1, (int64_t)((double)n * 0.6180339887));
+  auto gcd = [](int64_t x, int64_t y) {
+    while (y) {
+      const int64_t t = x % y;
+      x = y;
+      y = t;
+    }
+    return x;
+  };
+  while (gcd(m, n) != 1) ++m;
+  return m;
 }
 
 int launch_absmax(const float* pts, int64_t count, unsigned* out,
@@ -344,19 +402,21 @@ int cluster_tiles_async(const float* pts, int width, int height,
   a.k = k;
   a.cbuf = L.cbuf;
   a.totals = L.totals;
+  a.partials = L.partials;
   a.fcum = L.fcum;
   a.pstate = L.pstate;
   a.maxabs = L.maxabs;
   a.state = L.st;
   a.mbox = res->dev;
+  a.units_per_row = (a.tiles_x + 1) / 2;
+  a.nunits = (int64_t)a.units_per_row * ((height + kTileSide - 1) / kTileSide);
+  a.unit_mul = spread_multiplier(a.nunits);
 
   const int threads = tiles_threads(k);
   const int nwaves = threads / kWave;
   const int64_t want = (a.nseg + nwaves - 1) / nwaves;
-  const size_t lds =
-      std::max<size_t>((size_t)nwaves * 4 * k, (size_t)k + 2 * kWave) * 8 +
-      kMaxTileK * sizeof(float) +
-      (size_t)nwaves * 2 * kQueueCap * sizeof(int);
+  const size_t lds = (size_t)nwaves * 4 * k * 8 + kMaxTileK * sizeof(float) +
+                     (size_t)nwaves * 2 * kQueueCap * sizeof(int);
   // resident blocks per CU of the bounded kernel (its register use is the
   // larger of the two), so that the grid is one wave of blocks
   int per_cu = 1;
@@ -370,7 +430,8 @@ int cluster_tiles_async(const float* pts, int width, int height,
       per_cu = 1;
   }
   const int grid = (int)std::max<int64_t>(
-      1, std::min<int64_t>(want, (int64_t)device_cu_count() * per_cu));
+      1, std::min<int64_t>({want, (int64_t)device_cu_count() * per_cu,
+                            (int64_t)tiles_max_blocks(k)}));
   const bool vecmode = vec && (width % 4 == 0) &&
                        (((uintptr_t)L.pstate & 15u) == 0);
   auto launch = [&](bool full) {
@@ -381,6 +442,8 @@ int cluster_tiles_async(const float* pts, int width, int height,
       if (vecmode) tiles_launch_mode<false, kTileVec>(a, grid, threads, lds, stream);
       else tiles_launch_mode<false, kTile>(a, grid, threads, lds, stream);
     }
+    hipLaunchKernelGGL(tiles_reduce, dim3(1), dim3(kReduceThreads), 0, stream,
+                       a, grid);
   };
 
   constexpr int kChunk = 4;
