@@ -163,7 +163,7 @@ int segment_means(const int32_t* labels, const float* points, int64_t n,
                   float* centroids, int64_t k, void* temp, size_t temp_bytes,
                   hipStream_t stream);
 size_t tiles_workspace(int64_t n, int k);
-bool tiles_supported(int width, int k, int max_iter);
+bool tiles_supported(int64_t n, int width, int k);
 int cluster_tiles_async(const float* pts, int width, int height,
                         float* centroids, int k, int32_t* labels, void* temp,
                         size_t temp_bytes, int max_iter, float threshold,

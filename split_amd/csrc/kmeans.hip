@@ -560,7 +560,7 @@ int cluster_async(const float* pts, int64_t n, int width, int height,
   SPLIT_HIP_TRY(check_points(pts, n, k));
   if (!centroids) return fail(SPLIT_HIP_ERR_ARGUMENT, "centroids is null");
   if (!temp) return fail(SPLIT_HIP_ERR_ARGUMENT, "temp is null");
-  if (tiles_supported(width, k, max_iter) &&
+  if (tiles_supported(n, width, k) &&
       temp_bytes >= tiles_workspace(n, k))
     return cluster_tiles_async(pts, width, height, centroids, k, labels, temp,
                                temp_bytes, max_iter, threshold, stream,

@@ -24,10 +24,9 @@ struct TileArgs : Geometry {
   const unsigned* maxabs;     // bits of max |v| (non-negative float)
   LloydState* state;
   Mailbox* mbox;
-  // spread order of the bounded iterations (see SpreadUnits): units of two
-  // tiles, units per tile row, multiplier coprime to nunits
+  // bounded iterations of the single-image path: units (pairs of adjacent
+  // tiles) and the multiplier of their spread order
   int64_t nunits, unit_mul;
-  int units_per_row;
 };
 
 // Per-pixel state: label | budget, where budget = slack + F[label] at the
@@ -42,12 +41,11 @@ struct TileArgs : Geometry {
 // the history folded into one value per pixel).  The budget keeps the top
 // 24 bits of its float (sign, exponent, 15 mantissa bits); dropping the low
 // bits rounds a non-negative value down.
-// a + b (both >= 0) rounded down; x >= 0 to float rounded up.
+// A float <= a + b (both >= 0): the rounded sum, one ulp lower (rounding
+// to nearest is off by at most half an ulp); x >= 0 to float rounded up.
 __device__ __forceinline__ float add_down(float a, float b) {
   const float s = a + b;
-  return (double)s > (double)a + (double)b
-             ? __uint_as_float(__float_as_uint(s) - 1u)
-             : s;
+  return s > 0.f ? __uint_as_float(__float_as_uint(s) - 1u) : 0.f;
 }
 __device__ __forceinline__ float to_float_up(double x) {
   const float f = (float)x;
@@ -70,6 +68,10 @@ __device__ __forceinline__ float bound_step(float f, double d2j, int j,
   return to_float_up((double)f + (sqrt(d2j) + other) * (1.0 + 1e-5) + 1e-7);
 }
 
+// At most this many label changes in a wave are added to the slice with
+// per-pixel LDS atomics; more go through add_by_label's wave sums.
+constexpr int kAtomicChanges = 16;
+
 // Re-assigns up to 256 pixels (4 per lane: idx/v) whose previous labels are
 // old[] (-1: none), writes their new state, and accumulates: every pixel
 // (FULL) or the +/- changes of label (bounded iterations).
@@ -88,7 +90,8 @@ __device__ __forceinline__ void reassign(
   uint32_t ns[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
-    ns[i] = pack_state(lab[i], v[i] ? add_down(slack[i], fl[v[i] ? lab[i] : 0]) : 0.f);
+    ns[i] = pack_state(lab[i],
+                       add_down(v[i] ? slack[i] : 0.f, fl[v[i] ? lab[i] : 0]));
   if (lvec) {
     *reinterpret_cast<uint4*>(a.pstate + idx[0]) =
         make_uint4(ns[0], ns[1], ns[2], ns[3]);
@@ -122,16 +125,40 @@ __device__ __forceinline__ void reassign(
       add_by_label(slice, k, lane, lab, qx, qy, qz, 1);
     }
   } else {
-    bool any = false;
     int to[4], from[4];
+    uint64_t cb[4];
+    int nchg = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const bool chg = v[i] && lab[i] != old[i];
       to[i] = chg ? lab[i] : -1;
       from[i] = chg ? old[i] : -1;
-      any = any || chg;
+      cb[i] = ballot(chg);
+      nchg += __builtin_popcountll(cb[i]);
     }
-    if (ballot(any) != 0) {
+    if (nchg == 0) return;
+    if (nchg <= kAtomicChanges) {
+      // few changes: each changed pixel adds into the wave's slice with LDS
+      // atomics (the slice is the wave's own)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (cb[i] == 0) continue;
+        if (to[i] >= 0) {
+          unsigned long long* pt =
+              reinterpret_cast<unsigned long long*>(slice + to[i]);
+          unsigned long long* pf =
+              reinterpret_cast<unsigned long long*>(slice + from[i]);
+          atomicAdd(pt, (unsigned long long)(long long)qx[i]);
+          atomicAdd(pt + k, (unsigned long long)(long long)qy[i]);
+          atomicAdd(pt + 2 * k, (unsigned long long)(long long)qz[i]);
+          atomicAdd(pt + 3 * k, 1ull);
+          atomicAdd(pf, (unsigned long long)(-(long long)qx[i]));
+          atomicAdd(pf + k, (unsigned long long)(-(long long)qy[i]));
+          atomicAdd(pf + 2 * k, (unsigned long long)(-(long long)qz[i]));
+          atomicAdd(pf + 3 * k, ~0ull);
+        }
+      }
+    } else {
       add_by_label(slice, k, lane, to, qx, qy, qz, 1);
       add_by_label(slice, k, lane, from, qx, qy, qz, -1);
     }
@@ -144,38 +171,6 @@ struct TileRange {
   __device__ bool operator()(int64_t& t) {
     if (s >= end) return false;
     t = s++;
-    return true;
-  }
-};
-
-// Tile source for walk_tiles shared by the waves of one block: units of two
-// horizontally adjacent tiles (so a wave reads whole 128-byte rows of the
-// state), taken one at a time from a counter in LDS.  The block owns the
-// units p(r) for ranks r in [r0, r1), p(r) = r * mul mod nunits with mul
-// coprime to nunits, so that consecutive ranks land far apart in the image:
-// tiles that fail the skip test come in large patches, and spreading every
-// block's units over the whole image evens out the work per block.
-struct SpreadUnits {
-  int64_t r0, r1, nunits, mul;
-  int units_per_row, tiles_x;
-  unsigned* counter;  // LDS, zero before the first call
-  int lane;
-  int64_t pending;    // second tile of the current unit, or -1
-  __device__ bool operator()(int64_t& t) {
-    if (pending >= 0) {
-      t = pending;
-      pending = -1;
-      return true;
-    }
-    unsigned i = 0u;
-    if (lane == 0) i = atomicAdd(counter, 1u);
-    const int64_t r = r0 + (int64_t)(unsigned)readfirstlane_i((int)i);
-    if (r >= r1) return false;
-    const int64_t u = r * mul % nunits;
-    const int64_t row = u / units_per_row;
-    const int c = (int)(u - row * units_per_row);
-    t = row * tiles_x + 2 * c;
-    if (2 * c + 1 < tiles_x) pending = t + 1;
     return true;
   }
 };

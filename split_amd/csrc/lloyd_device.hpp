@@ -219,8 +219,10 @@ __device__ __forceinline__ void assign_segment(
     for (int i = 0; i < 4; ++i) {
       // squared lower bound of the runner-up, then a 2e-5 relative shrink
       // (and growth of the winner) far above the float rounding involved
+      // (hardware square root, within 1 ulp: far inside the 2e-5 shrink)
       const float l2 = fminf(fminf(bd2[i], lbox), bd[i] + 2.f * fmaxf(mbis, 0.f));
-      const float s = sqrtf(l2 * (1.f - 2e-5f)) - sqrtf(bd[i] * (1.f + 2e-5f));
+      const float s = __builtin_amdgcn_sqrtf(l2 * (1.f - 2e-5f)) -
+                      __builtin_amdgcn_sqrtf(bd[i] * (1.f + 2e-5f));
       slack[i] = s > 0.f ? s : 0.f;
     }
   }

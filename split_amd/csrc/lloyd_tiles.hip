@@ -34,61 +34,72 @@
 namespace splithip {
 
 // totals (tot, [4][k]) -> centroids, shift, bound consumption, stop
-// decision; one block.
+// decision; one block of at least k threads, thread j < k holding the old
+// centroid j (oc) and F[j] (f).
 __device__ void tiles_finalize(const TileArgs& a, const long long* tot,
-                               const float* cent, float* cent_out,
-                               float fscale, double* scratch) {
+                               const float (&oc)[3], float f, float* cent_out,
+                               float fscale, double* d2) {
   const int k = a.k;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  double* d2 = scratch;             // k entries: |new - old|^2
-  double* part = scratch + k;       // kWave partial sums
+  const int tid = threadIdx.x;
   const double inv = 1.0 / (double)fscale;
-  for (int j = tid; j < k; j += nt) {
-    const long long cnt = tot[3 * k + j];
+  if (tid < k) {
+    const long long cnt = tot[3 * k + tid];
     double s = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const long long sum = tot[c * k + j];
+      const long long sum = tot[c * k + tid];
       const float nc = cnt > 0 ? (float)((double)sum * inv / (double)cnt) : 0.f;
-      const double d = (double)nc - (double)cent[c * k + j];
+      const double d = (double)nc - (double)oc[c];
       s += d * d;
-      cent_out[c * k + j] = nc;
+      cent_out[c * k + tid] = nc;
     }
-    d2[j] = s;
-  }
-  __syncthreads();
-  if (tid < kWave) {
-    double s = 0.0;
-    for (int j = tid; j < k; j += kWave) s += d2[j];
-    part[tid] = s;
+    d2[tid] = s;
   }
   __syncthreads();
   __shared__ double top[2];
   __shared__ int top_arg;
   __shared__ float shift_sh;
-  if (tid == 0) {
+  if (tid < kWave) {
+    // shift: lane l sums d2[l], d2[l + 64], ...; the 64 partial sums are
+    // then added in lane order
+    double part = 0.0;
+    for (int j = tid; j < k; j += kWave) part += d2[j];
+    const uint64_t pb = (uint64_t)__double_as_longlong(part);
+    const uint32_t plo = (uint32_t)pb, phi = (uint32_t)(pb >> 32);
     double s = 0.0;
-    for (int j = 0; j < kWave; ++j) s += part[j];
-    shift_sh = (float)sqrt(s);
-    double m1 = 0.0, m2 = 0.0;
-    int arg = -1;
-    for (int j = 0; j < k; ++j) {
-      const double d = d2[j];
-      if (d > m1) {
-        m2 = m1;
-        m1 = d;
+#pragma unroll
+    for (int l = 0; l < kWave; ++l)
+      s += __longlong_as_double(
+          (long long)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)phi, l)
+                       << 32) |
+                      (uint32_t)__builtin_amdgcn_readlane((int)plo, l)));
+    // largest and second-largest squared move; arg = lowest index of the
+    // largest, m2 = largest over the other indices
+    double m1 = -1.0;  // moves are >= 0: the lane's first index is taken
+    int arg = 0x7fffffff;
+    for (int j = tid; j < k; j += kWave)
+      if (d2[j] > m1) {
+        m1 = d2[j];
         arg = j;
-      } else if (d > m2) {
-        m2 = d;
       }
+    double g1 = m1;
+    for (int o = kWave / 2; o > 0; o >>= 1) g1 = fmax(g1, __shfl_xor(g1, o));
+    int ga = m1 == g1 ? arg : 0x7fffffff;
+    for (int o = kWave / 2; o > 0; o >>= 1) ga = min(ga, __shfl_xor(ga, o));
+    double m2 = 0.0;
+    for (int j = tid; j < k; j += kWave)
+      if (j != ga) m2 = fmax(m2, d2[j]);
+    for (int o = kWave / 2; o > 0; o >>= 1) m2 = fmax(m2, __shfl_xor(m2, o));
+    if (tid == 0) {
+      shift_sh = (float)sqrt(s);
+      top[0] = g1;
+      top[1] = m2;
+      top_arg = ga;
     }
-    top[0] = m1;
-    top[1] = m2;
-    top_arg = arg;
   }
   __syncthreads();
-  for (int j = tid; j < k; j += nt)
-    a.fcum[j] = bound_step(a.fcum[j], d2[j], j, top[0], top[1], top_arg);
+  if (tid < k)
+    a.fcum[tid] = bound_step(f, d2[tid], tid, top[0], top[1], top_arg);
   if (tid == 0) {
     LloydState* st = a.state;
     const float shift = shift_sh;
@@ -98,15 +109,207 @@ __device__ void tiles_finalize(const TileArgs& a, const long long* tot,
     st->shift = shift;
     st->cur = 1 - st->cur;
     st->done = done;
-    if (a.mbox) {
+    if (a.mbox) {  // read by the host after it synchronises with the launch
       volatile Mailbox* mb = a.mbox;
       mb->iter = it;
       mb->shift = shift;
       mb->done = done;
-      __threadfence_system();
     }
   }
 }
+
+// ------------------------------------------------ bounded iterations ---
+//
+// Units are pairs of horizontally adjacent tiles, ranked in a spread order:
+// rank r is unit r * mul mod nunits (units row-major, mul coprime to
+// nunits), so that consecutive ranks land far apart in the image.  Tiles
+// that fail the skip test come in large patches; spreading every block's
+// units over the whole image evens out the work per block.  A block owns
+// the ranks [r0, r1); per round of up to kBlockUnits units it lists them in
+// LDS as {offset of the first pixel, first row} and its waves take them
+// through an LDS counter.  Pixel offsets are int32 (the tiled path
+// requires n < 2^31).
+constexpr int kBlockUnits = 2048;
+
+// One tile: offset of its first pixel, first row and column.
+struct TileGeo {
+  int off, row0, col0;
+};
+
+// Fills list[0, cnt) with the units of ranks q0 .. q0 + cnt - 1.
+__device__ __forceinline__ void list_units(const TileArgs& a, int64_t q0,
+                                           int cnt, int2* list) {
+  const int upr = (a.tiles_x + 1) / 2;
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+    const int64_t u = (q0 + i) * a.unit_mul % a.nunits;
+    const int ty = (int)(u / upr);
+    const int c = (int)(u - (int64_t)ty * upr);
+    const int row0 = ty * kTileSide;
+    list[i] = make_int2(row0 * a.width + 2 * c * kTileSide, row0);
+  }
+}
+
+template <int CPL, int MODE>
+struct BoundedWalker {
+  const TileArgs& a;
+  const float (&cx)[CPL];
+  const float (&cy)[CPL];
+  const float (&cz)[CPL];
+  const float (&cn)[CPL];
+  const float* fl;
+  float fscale, margin;
+  long long* slice;
+  int* qidx;
+  int* qold;
+  int lane, lane_off;  // lane l: row l/4, columns 4(l%4)..+3 of a tile
+  IRun run;
+  int qlen;            // wave-uniform
+
+  __device__ void tile_pixels(const TileGeo& g, int64_t (&idx)[4],
+                              bool (&v)[4], bool& lvec) const {
+    const int col = g.col0 + 4 * (lane & 3);
+    const bool rowok = g.row0 + (lane >> 2) < a.height;
+    const int o = g.off + lane_off;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = rowok && col + i < a.width;
+      idx[i] = o + i;
+    }
+    lvec = MODE == kTileVec && rowok && col + 3 < a.width;
+  }
+
+  __device__ void load_state(const TileGeo& g, bool have,
+                             uint32_t (&ps)[4]) const {
+    const int col = g.col0 + 4 * (lane & 3);
+    const bool rowok = have && g.row0 + (lane >> 2) < a.height;
+    const uint32_t* p = a.pstate + (have ? g.off + lane_off : 0);
+    if (MODE == kTileVec && rowok && col + 3 < a.width) {
+      const uint4 q = *reinterpret_cast<const uint4*>(p);
+      ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        ps[i] = rowok && col + i < a.width ? p[i] : 0u;
+    }
+  }
+
+  // Re-assigns the first min(qlen, 256) queued pixels, then moves the rest
+  // to the front.
+  __device__ void drain() {
+    const int c = qlen < 256 ? qlen : 256;
+    int64_t idx[4];
+    bool v[4];
+    int old[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = lane + 64 * i;
+      v[i] = e < c;
+      idx[i] = qidx[v[i] ? e : 0];
+      old[i] = v[i] ? qold[e] : -1;
+    }
+    reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, false,
+                         old, slice, run);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = 256 + lane + 64 * i;
+      if (e < qlen) {
+        qidx[e - 256] = qidx[e];
+        qold[e - 256] = qold[e];
+      }
+    }
+    qlen -= c;
+  }
+
+  // Skip test of one tile; re-assigns it whole or queues its failing pixels.
+  __device__ void visit(const TileGeo& g, const uint32_t (&ps)[4]) {
+    const int col = g.col0 + 4 * (lane & 3);
+    const bool rowok = g.row0 + (lane >> 2) < a.height;
+    int old[4];
+    bool fail[4];
+    uint64_t fb[4];
+    int nfail = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bool v = rowok && col + i < a.width;
+      old[i] = (int)(ps[i] & 0xFFu);
+      fail[i] = v && !(state_budget(ps[i]) - fl[old[i]] > margin);
+      fb[i] = ballot(fail[i]);
+      nfail += __builtin_popcountll(fb[i]);
+    }
+    if (nfail == 0) return;  // every pixel provably unchanged
+    if (nfail > kWholeTileFails) {
+      int64_t idx[4];
+      bool v[4], lvec;
+      tile_pixels(g, idx, v, lvec);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) old[i] = v[i] ? old[i] : -1;
+      reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, lvec,
+                           old, slice, run);
+      return;
+    }
+    // queue the failing pixels (in lane order within each slot)
+    const int o = g.off + lane_off;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(fb[i] >> 32),
+          __builtin_amdgcn_mbcnt_lo((uint32_t)fb[i], 0u));
+      if (fail[i]) {
+        qidx[qlen + below] = o + i;
+        qold[qlen + below] = old[i];
+      }
+      qlen += __builtin_popcountll(fb[i]);
+    }
+    if (qlen >= 256) drain();
+  }
+
+  // Walks the units list[0, cnt), taken through *counter (zero on entry),
+  // loading the state of the next tile while the current one is tested.
+  __device__ void walk(const int2* list, int cnt, unsigned* counter) {
+    int pend_off = 0, pend_row0 = 0, pend_col0 = 0;
+    bool pending = false;
+    auto next = [&](TileGeo& g) {
+      if (pending) {
+        pending = false;
+        g = TileGeo{pend_off, pend_row0, pend_col0};
+        return true;
+      }
+      unsigned i = 0u;
+      if (lane == 0) i = atomicAdd(counter, 1u);
+      const int u = readfirstlane_i((int)i);
+      if (u >= cnt) return false;
+      const int2 e = list[u];
+      const int col0 = e.x - e.y * a.width;
+      g = TileGeo{e.x, e.y, col0};
+      if (col0 + kTileSide < a.width) {
+        pending = true;
+        pend_off = e.x + kTileSide;
+        pend_row0 = e.y;
+        pend_col0 = col0 + kTileSide;
+      }
+      return true;
+    };
+    TileGeo g{0, 0, 0};
+    bool have = next(g);
+    uint32_t ps[4];
+    load_state(g, have, ps);
+    while (have) {
+      TileGeo g2{0, 0, 0};
+      const bool have2 = next(g2);
+      uint32_t ps2[4];
+      load_state(g2, have2, ps2);
+      visit(g, ps);
+      g = g2;
+      have = have2;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ps[i] = ps2[i];
+    }
+  }
+
+  __device__ void finish() {
+    while (qlen > 0) drain();
+  }
+};
 
 template <int CPL, int MODE, bool FULL>
 __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
@@ -126,8 +329,6 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
   int* qold = qidx + kQueueCap;
   for (int i = threadIdx.x; i < nwaves * V; i += blockDim.x) slices[i] = 0;
   for (int i = threadIdx.x; i < k; i += blockDim.x) fl[i] = a.fcum[i];
-  __shared__ unsigned unit_next;
-  if (threadIdx.x == 0) unit_next = 0u;
 
   float cx[CPL], cy[CPL], cz[CPL], cn[CPL];
 #pragma unroll
@@ -154,13 +355,25 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
         a, TileRange{a.nseg * gw / W, a.nseg * (gw + 1) / W}, cx, cy, cz, cn,
         fl, fscale, margin, slice, qidx, qold, lane);
   } else {
+    int2* list = reinterpret_cast<int2*>(
+        reinterpret_cast<int*>(fl + kMaxTileK) + nwaves * 2 * kQueueCap);
+    __shared__ unsigned unit_next;
+    BoundedWalker<CPL, MODE> w{a,     cx,    cy,   cz,   cn,
+                               fl,    fscale, margin, slice, qidx,
+                               qold,  lane,  (lane >> 2) * a.width + 4 * (lane & 3),
+                               IRun{0, 0, 0, 0, -1, 0}, 0};
     const int64_t G = gridDim.x;
-    walk_tiles<CPL, MODE, FULL>(
-        a,
-        SpreadUnits{a.nunits * blockIdx.x / G, a.nunits * (blockIdx.x + 1) / G,
-                    a.nunits, a.unit_mul, a.units_per_row, a.tiles_x,
-                    &unit_next, lane, -1},
-        cx, cy, cz, cn, fl, fscale, margin, slice, qidx, qold, lane);
+    const int64_t r0 = a.nunits * blockIdx.x / G;
+    const int64_t r1 = a.nunits * (blockIdx.x + 1) / G;
+    for (int64_t q0 = r0; q0 < r1; q0 += kBlockUnits) {
+      const int cnt = (int)std::min<int64_t>(kBlockUnits, r1 - q0);
+      list_units(a, q0, cnt, list);
+      if (threadIdx.x == 0) unit_next = 0u;
+      __syncthreads();
+      w.walk(list, cnt, &unit_next);
+      __syncthreads();  // the list is refilled by the next round
+    }
+    w.finish();
   }
   // the block's share of the step, summed over its waves; tiles_reduce adds
   // the shares of all blocks into the totals after this launch ends
@@ -181,31 +394,52 @@ __global__ __launch_bounds__(kReduceThreads) void tiles_reduce(TileArgs a,
                                                                 int blocks) {
   __shared__ long long red[kReduceThreads];
   __shared__ long long tot[4 * kMaxTileK];
-  __shared__ double scratch[kMaxTileK + kWave];
+  __shared__ double d2[kMaxTileK];
   LloydState* st = a.state;
   if (st->done) return;
   const int k = a.k, V = 4 * k;
   const int tid = threadIdx.x;
   static_assert(4 * kMaxTileK <= kReduceThreads, "one pass over the entries");
-  // thread t sums entry t % V over blocks t / V, t / V + per, ...
+  static_assert(kMaxTileK <= kReduceThreads, "one centroid per thread");
+  const int cur = st->cur;
+  const float* cent = a.cbuf + (size_t)cur * 3 * k;
+  float* cent_out = a.cbuf + (size_t)(1 - cur) * 3 * k;
+  float oc[3] = {0.f, 0.f, 0.f}, f = 0.f;
+  if (tid < k) {
+    oc[0] = cent[tid];
+    oc[1] = cent[k + tid];
+    oc[2] = cent[2 * k + tid];
+    f = a.fcum[tid];
+  }
+  const long long before = tid < V ? a.totals[tid] : 0;
+  // thread t sums entry t % V over blocks t / V, t / V + per, ...: loads
+  // in batches of kBatch in flight
+  constexpr int kBatch = 32;
   const int per = kReduceThreads / V;
   const int v = tid % V, q = tid / V;
   long long sum = 0;
-  if (q < per)
-    for (int g = q; g < blocks; g += per) sum += a.partials[(size_t)g * V + v];
+  if (q < per) {
+    for (int g0 = q; g0 < blocks; g0 += per * kBatch) {
+      long long x[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int g = g0 + u * per;
+        x[u] = g < blocks ? a.partials[(size_t)g * V + v] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) sum += x[u];
+    }
+  }
   red[tid] = sum;
   __syncthreads();
   if (tid < V) {
-    long long t = a.totals[tid];
+    long long t = before;
     for (int r = 0; r < per; ++r) t += red[r * V + tid];
     a.totals[tid] = t;
     tot[tid] = t;
   }
   __syncthreads();
-  const int cur = st->cur;
-  const float* cent = a.cbuf + (size_t)cur * 3 * k;
-  float* cent_out = a.cbuf + (size_t)(1 - cur) * 3 * k;
-  tiles_finalize(a, tot, cent, cent_out, scale_of(*a.maxabs), scratch);
+  tiles_finalize(a, tot, oc, f, cent_out, scale_of(*a.maxabs), d2);
 }
 
 __global__ void tiles_init(LloydState* st, float* cbuf, const float* cent,
@@ -281,6 +515,13 @@ static int tiles_threads(int k) { return k <= 128 ? 1024 : 256; }
 // Step launches use at most this many blocks (partials in temp are sized
 // for it).
 static int tiles_max_blocks(int k) { return k <= 128 ? 512 : 1024; }
+// Units (pairs of adjacent tiles) of a width x (n / width) image.
+static int64_t tile_units(int64_t n, int width) {
+  if (width <= 0) return 0;
+  const int64_t tiles_x = (width + kTileSide - 1) / kTileSide;
+  const int64_t tiles_y = (n / width + kTileSide - 1) / kTileSide;
+  return (tiles_x + 1) / 2 * tiles_y;
+}
 
 struct TileLayout {
   LloydState* st;
@@ -344,9 +585,9 @@ size_t tiles_workspace(int64_t n, int k) {
   return tiles_layout(nullptr, n, k).bytes;
 }
 
-bool tiles_supported(int width, int k, int max_iter) {
-  (void)max_iter;
-  return width > 0 && k >= 1 && k <= kMaxTileK;
+bool tiles_supported(int64_t n, int width, int k) {
+  // pixel indices are listed as int32
+  return width > 0 && k >= 1 && k <= kMaxTileK && n <= 0x7fffffff;
 }
 
 template <bool FULL, int MODE>
@@ -408,15 +649,15 @@ int cluster_tiles_async(const float* pts, int width, int height,
   a.maxabs = L.maxabs;
   a.state = L.st;
   a.mbox = res->dev;
-  a.units_per_row = (a.tiles_x + 1) / 2;
-  a.nunits = (int64_t)a.units_per_row * ((height + kTileSide - 1) / kTileSide);
+  a.nunits = tile_units(n, width);
   a.unit_mul = spread_multiplier(a.nunits);
 
   const int threads = tiles_threads(k);
   const int nwaves = threads / kWave;
   const int64_t want = (a.nseg + nwaves - 1) / nwaves;
   const size_t lds = (size_t)nwaves * 4 * k * 8 + kMaxTileK * sizeof(float) +
-                     (size_t)nwaves * 2 * kQueueCap * sizeof(int);
+                     (size_t)nwaves * 2 * kQueueCap * sizeof(int) +
+                     kBlockUnits * sizeof(int2);
   // resident blocks per CU of the bounded kernel (its register use is the
   // larger of the two), so that the grid is one wave of blocks
   int per_cu = 1;
