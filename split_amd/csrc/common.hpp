@@ -135,12 +135,13 @@ struct LloydState {
 
 // Host-mapped mailbox the last block of a step launch writes, so the host
 // can stop enqueuing iterations without a device->host copy.
-struct Mailbox {
+struct alignas(16) Mailbox {
   int iter;
   int done;
   float shift;
   int reserved;
 };
+static_assert(sizeof(Mailbox) == 16, "written as one 16-byte store");
 
 // Per-thread resources of the Lloyd drivers: a host-mapped mailbox and two
 // events, created on first use and kept for the life of the thread.

@@ -18,7 +18,7 @@ struct TileArgs : Geometry {
   int k;
   float* cbuf;                // two 3k centroid buffers
   long long* totals;          // [4][k]: quantised channel sums, counts
-  long long* partials;        // [blocks][4k]: each block's share of a step
+  long long* partials;        // step accumulator copies, [copies][4k]
   float* fcum;                // per-centroid bound consumption F[j], k
   uint32_t* pstate;           // per-pixel state (see pack_state)
   const unsigned* maxabs;     // bits of max |v| (non-negative float)

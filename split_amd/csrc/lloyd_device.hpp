@@ -119,11 +119,20 @@ __device__ __forceinline__ void assign_segment(
   // Colour box of the segment's valid pixels.
   float lx = INFINITY, ly = INFINITY, lz = INFINITY;
   float hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
+  if (ballot(!(v[0] && v[1] && v[2] && v[3])) == 0) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    lx = v[i] ? fminf(lx, x[i]) : lx;  hx = v[i] ? fmaxf(hx, x[i]) : hx;
-    ly = v[i] ? fminf(ly, y[i]) : ly;  hy = v[i] ? fmaxf(hy, y[i]) : hy;
-    lz = v[i] ? fminf(lz, z[i]) : lz;  hz = v[i] ? fmaxf(hz, z[i]) : hz;
+    for (int i = 0; i < 4; ++i) {
+      lx = fminf(lx, x[i]);  hx = fmaxf(hx, x[i]);
+      ly = fminf(ly, y[i]);  hy = fmaxf(hy, y[i]);
+      lz = fminf(lz, z[i]);  hz = fmaxf(hz, z[i]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      lx = v[i] ? fminf(lx, x[i]) : lx;  hx = v[i] ? fmaxf(hx, x[i]) : hx;
+      ly = v[i] ? fminf(ly, y[i]) : ly;  hy = v[i] ? fmaxf(hy, y[i]) : hy;
+      lz = v[i] ? fminf(lz, z[i]) : lz;  hz = v[i] ? fmaxf(hz, z[i]) : hz;
+    }
   }
   lx = wave_allreduce(lx, OpMin()); hx = wave_allreduce(hx, OpMax());
   ly = wave_allreduce(ly, OpMin()); hy = wave_allreduce(hy, OpMax());

@@ -109,12 +109,9 @@ __device__ void tiles_finalize(const TileArgs& a, const long long* tot,
     st->shift = shift;
     st->cur = 1 - st->cur;
     st->done = done;
-    if (a.mbox) {  // read by the host after it synchronises with the launch
-      volatile Mailbox* mb = a.mbox;
-      mb->iter = it;
-      mb->shift = shift;
-      mb->done = done;
-    }
+    if (a.mbox)  // one 16-byte store; the host reads it after a sync
+      *reinterpret_cast<int4*>(a.mbox) =
+          make_int4(it, done, __float_as_int(shift), 0);
   }
 }
 
@@ -130,6 +127,7 @@ __device__ void tiles_finalize(const TileArgs& a, const long long* tot,
 // through an LDS counter.  Pixel offsets are int32 (the tiled path
 // requires n < 2^31).
 constexpr int kBlockUnits = 2048;
+constexpr int kMaxQueueWaves = 16;  // waves per block (1024 threads)
 
 // One tile: offset of its first pixel, first row and column.
 struct TileGeo {
@@ -224,13 +222,15 @@ struct BoundedWalker {
   __device__ void visit(const TileGeo& g, const uint32_t (&ps)[4]) {
     const int col = g.col0 + 4 * (lane & 3);
     const bool rowok = g.row0 + (lane >> 2) < a.height;
+    const bool interior = g.row0 + kTileSide <= a.height &&
+                          g.col0 + kTileSide <= a.width;  // wave-uniform
     int old[4];
     bool fail[4];
     uint64_t fb[4];
     int nfail = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const bool v = rowok && col + i < a.width;
+      const bool v = interior || (rowok && col + i < a.width);
       old[i] = (int)(ps[i] & 0xFFu);
       fail[i] = v && !(state_budget(ps[i]) - fl[old[i]] > margin);
       fb[i] = ballot(fail[i]);
@@ -306,10 +306,52 @@ struct BoundedWalker {
     }
   }
 
-  __device__ void finish() {
-    while (qlen > 0) drain();
+  // After the walk: the waves' queues (fewer than 256 pixels each) are
+  // re-assigned as one list, their concatenation in wave order, in batches
+  // of 256 handed out through an LDS counter.  queues = the first wave's
+  // queue (each wave's spans 2 * kQueueCap ints: indices, then labels).
+  __device__ void finish_block(int* queues, int wave, int nwaves) {
+    __shared__ int start[kMaxQueueWaves + 1];
+    __shared__ unsigned next_batch;
+    if (lane == 0) start[wave + 1] = qlen;
+    if (threadIdx.x == 0) {
+      start[0] = 0;
+      next_batch = 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int w = 1; w <= nwaves; ++w) start[w] += start[w - 1];
+    __syncthreads();
+    const int total = start[nwaves];
+    for (;;) {
+      unsigned b = 0u;
+      if (lane == 0) b = atomicAdd(&next_batch, 1u);
+      const int e0 = 256 * readfirstlane_i((int)b);
+      if (e0 >= total) break;
+      int64_t idx[4];
+      bool v[4];
+      int old[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int pos = e0 + lane + 64 * i;
+        v[i] = pos < total;
+        int w = 0;
+        for (int j = 1; j < nwaves; ++j) w += pos >= start[j] ? 1 : 0;
+        const int* q = queues + w * 2 * kQueueCap;
+        const int e = v[i] ? pos - start[w] : 0;
+        idx[i] = v[i] ? q[e] : 0;
+        old[i] = v[i] ? q[kQueueCap + e] : -1;
+      }
+      reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, false,
+                           old, slice, run);
+    }
+    qlen = 0;
   }
 };
+
+// Copies of the step accumulator (blocks add into copy blockIdx % 8, so
+// that fewer blocks contend for one address).
+constexpr int kAccCopies = 8;
 
 template <int CPL, int MODE, bool FULL>
 __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
@@ -373,33 +415,34 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
       w.walk(list, cnt, &unit_next);
       __syncthreads();  // the list is refilled by the next round
     }
-    w.finish();
+    w.finish_block(reinterpret_cast<int*>(fl + kMaxTileK), wave, nwaves);
   }
-  // the block's share of the step, summed over its waves; tiles_reduce adds
-  // the shares of all blocks into the totals after this launch ends
+  // the block's share of the step, summed over its waves, is added (device
+  // atomics) into copy blockIdx % kAccCopies of the step accumulator;
+  // tiles_reduce adds the copies into the totals after this launch ends
   __syncthreads();
-  long long* part = a.partials + (size_t)blockIdx.x * V;
+  long long* acc = a.partials + (size_t)(blockIdx.x % kAccCopies) * V;
   for (int vv = threadIdx.x; vv < V; vv += blockDim.x) {
     long long tot = 0;
     for (int w = 0; w < nwaves; ++w) tot += slices[(size_t)w * V + vv];
-    part[vv] = tot;
+    if (tot != 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(acc + vv),
+                (unsigned long long)tot);
   }
 }
 
-// One block after each step launch: totals += the shares of the step's
-// `blocks` blocks (fixed order; integer sums), then tiles_finalize.
+// One block after each step launch: totals += the accumulator copies
+// (integer sums), which it clears for the next step, then tiles_finalize.
 constexpr int kReduceThreads = 1024;
 
-__global__ __launch_bounds__(kReduceThreads) void tiles_reduce(TileArgs a,
-                                                                int blocks) {
-  __shared__ long long red[kReduceThreads];
+__global__ __launch_bounds__(kReduceThreads) void tiles_reduce(TileArgs a) {
   __shared__ long long tot[4 * kMaxTileK];
   __shared__ double d2[kMaxTileK];
   LloydState* st = a.state;
   if (st->done) return;
   const int k = a.k, V = 4 * k;
   const int tid = threadIdx.x;
-  static_assert(4 * kMaxTileK <= kReduceThreads, "one pass over the entries");
+  static_assert(4 * kMaxTileK <= kReduceThreads, "one entry per thread");
   static_assert(kMaxTileK <= kReduceThreads, "one centroid per thread");
   const int cur = st->cur;
   const float* cent = a.cbuf + (size_t)cur * 3 * k;
@@ -411,30 +454,16 @@ __global__ __launch_bounds__(kReduceThreads) void tiles_reduce(TileArgs a,
     oc[2] = cent[2 * k + tid];
     f = a.fcum[tid];
   }
-  const long long before = tid < V ? a.totals[tid] : 0;
-  // thread t sums entry t % V over blocks t / V, t / V + per, ...: loads
-  // in batches of kBatch in flight
-  constexpr int kBatch = 32;
-  const int per = kReduceThreads / V;
-  const int v = tid % V, q = tid / V;
-  long long sum = 0;
-  if (q < per) {
-    for (int g0 = q; g0 < blocks; g0 += per * kBatch) {
-      long long x[kBatch];
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const int g = g0 + u * per;
-        x[u] = g < blocks ? a.partials[(size_t)g * V + v] : 0;
-      }
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) sum += x[u];
-    }
-  }
-  red[tid] = sum;
-  __syncthreads();
   if (tid < V) {
-    long long t = before;
-    for (int r = 0; r < per; ++r) t += red[r * V + tid];
+    long long t = a.totals[tid];
+    long long x[kAccCopies];
+#pragma unroll
+    for (int c = 0; c < kAccCopies; ++c) x[c] = a.partials[(size_t)c * V + tid];
+#pragma unroll
+    for (int c = 0; c < kAccCopies; ++c) {
+      t += x[c];
+      a.partials[(size_t)c * V + tid] = 0;  // for the next step
+    }
     a.totals[tid] = t;
     tot[tid] = t;
   }
@@ -446,7 +475,8 @@ __global__ void tiles_init(LloydState* st, float* cbuf, const float* cent,
                            int k, int max_iter, float threshold,
                            long long* totals, float* fcum, unsigned* maxabs) {
   for (int i = threadIdx.x; i < 3 * k; i += blockDim.x) cbuf[i] = cent[i];
-  for (int i = threadIdx.x; i < 4 * k; i += blockDim.x) totals[i] = 0;
+  for (int i = threadIdx.x; i < 4 * k * (1 + kAccCopies); i += blockDim.x)
+    totals[i] = 0;  // totals, then the step accumulator copies
   for (int i = threadIdx.x; i < k; i += blockDim.x) fcum[i] = 0.f;
   if (threadIdx.x == 0) {
     st->cur = 0;
@@ -512,8 +542,7 @@ __global__ void tiles_finish(const LloydState* st, const float* cbuf,
 // waves of a CU then share one pool of tiles in the bounded iterations and
 // finish together.
 static int tiles_threads(int k) { return k <= 128 ? 1024 : 256; }
-// Step launches use at most this many blocks (partials in temp are sized
-// for it).
+// Step launches use at most this many blocks.
 static int tiles_max_blocks(int k) { return k <= 128 ? 512 : 1024; }
 // Units (pairs of adjacent tiles) of a width x (n / width) image.
 static int64_t tile_units(int64_t n, int width) {
@@ -545,9 +574,10 @@ static TileLayout tiles_layout(void* temp, int64_t n, int k) {
   };
   L.st = reinterpret_cast<LloydState*>(take(sizeof(LloydState)));
   L.cbuf = reinterpret_cast<float*>(take(6 * (size_t)k * sizeof(float)));
-  L.totals = reinterpret_cast<long long*>(take(4 * (size_t)k * 8));
-  L.partials = reinterpret_cast<long long*>(
-      take((size_t)tiles_max_blocks(k) * 4 * k * 8));
+  // totals and the kAccCopies accumulator copies, contiguous
+  L.totals = reinterpret_cast<long long*>(
+      take((size_t)(1 + kAccCopies) * 4 * k * 8));
+  L.partials = L.totals ? L.totals + 4 * (size_t)k : nullptr;
   L.fcum = reinterpret_cast<float*>(take((size_t)k * sizeof(float)));
   L.maxabs = reinterpret_cast<unsigned*>(take(sizeof(unsigned)));
   L.pstate = reinterpret_cast<uint32_t*>(take((size_t)n * sizeof(uint32_t)));
@@ -684,7 +714,7 @@ int cluster_tiles_async(const float* pts, int width, int height,
       else tiles_launch_mode<false, kTile>(a, grid, threads, lds, stream);
     }
     hipLaunchKernelGGL(tiles_reduce, dim3(1), dim3(kReduceThreads), 0, stream,
-                       a, grid);
+                       a);
   };
 
   constexpr int kChunk = 4;
