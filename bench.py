@@ -7,18 +7,23 @@ workload: shaded_4096 (4096x4096 Voronoi x Lambertian shading, k=32, stop at
 centroid shift < 1e-4 or 100 iterations, 8-connectivity).
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the bounded
-Lloyd step, tiles_step): 12 bytes/pixel algorithmic traffic (three float
-planes) per launch, over its average launch time measured with HIP events on
-the stream it runs on, as the slope of cluster() time between two fixed
-iteration counts (8 and 40).  `traffic` is the HBM bytes per launch of the
-same launches, from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) of
+Lloyd iteration: a tiles_step launch and the one-block tiles_reduce after
+it): 12 bytes/pixel algorithmic traffic (three float planes) per iteration,
+over its average time measured with HIP events on the stream it runs on, as
+the slope of cluster() time between two fixed iteration counts (8 and 40).
+`traffic` is the HBM bytes per bounded tiles_step launch of the same
+iterations, from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) of
 `bench.py --profile-driver`, scaled by the counters of a 256 MiB copy.
 `cpu_baseline` times the oracle's restatement of the reference pipeline on a
 bounded crop of the same image, single-threaded, on this host.
 
 `--workload batch_256` instead runs the batched path (256 images of uneven
 size and k, one graph-replayed launch sequence) against the same images
-through 256 separate segment calls.
+through 256 separate segment calls.  Other workloads: cc_stress_8192 (CC +
+compress of the spiral/checkerboard labels), segmeans_4096 (segment means +
+propagation), color_4096 (Lab + beta features), ingest_4096 (host image
+upload / download / 8-bit egress), and the image workloads of
+split_amd.workloads (plain_512, stress_8192, ...).
 """
 from __future__ import annotations
 
@@ -317,6 +322,149 @@ def segmeans_bench(args):
     }), flush=True)
 
 
+def _events_ms(fn, steps, warmup):
+    """Average ms of fn() over `steps` calls on the current stream (HIP
+    events), after `warmup` calls."""
+    import torch
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def color_bench(args):
+    """The features k-means clusters in the reference's sample (main.cu:216-225):
+    rgb -> Lab, copy of L, beta over L, on the shaded_4096 image.  The fused
+    split_hip_lab_beta_feature against the separate calls (convert, copy,
+    beta_feature)."""
+    import torch
+    import split_amd as sa
+    from split_amd import workloads as W
+    w = W.SHADED_4096
+    n = w.n
+    P = torch.from_numpy(W.make_image(w.image)).cuda()
+    F = torch.empty_like(P)
+    lum = torch.empty(n, dtype=torch.float32, device="cuda")
+    temp = torch.empty(sa.lib().split_hip_beta_feature_workspace(), dtype=torch.uint8,
+                       device="cuda")
+    ms_fused = _events_ms(lambda: sa.lab_beta_feature(P, F, lum, 1e5, temp=temp),
+                          args.steps * 10, args.warmup)
+
+    def separate():
+        sa.convert_color_space(P, F, "rgb_to_lab")
+        lum.copy_(F[0])
+        sa.beta_feature(F, F[0], 1e5, temp=temp)
+
+    ms_sep = _events_ms(separate, args.steps * 10, args.warmup)
+    algo = 28 * n  # read 3 planes, write 3 feature planes and the luminance copy
+    print(json.dumps({
+        "metric": "Lab + beta feature Mpix/s (the k-means input features)",
+        "value": round(n / (ms_fused / 1e3) / 1e6, 3), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": args.steps * 10, "warmup": args.warmup, "ms_per_step": round(ms_fused, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": "color_4096", "image": "4096x4096", "mu": 1e5,
+                   "parallelism": "single GPU"},
+        "algorithmic_bytes_per_step": algo,
+        "hbm_frac": round(algo / (ms_fused / 1e3) / (HBM_PEAK_GBS * 1e9), 4),
+        "designed_bytes_per_step": 44 * n,
+        "separate_calls": {"ms_per_step": round(ms_sep, 4),
+                           "value": round(n / (ms_sep / 1e3) / 1e6, 3),
+                           "speedup_fused": round(ms_sep / ms_fused, 3)},
+    }), flush=True)
+
+
+def ingest_bench(args):
+    """Host ingest / egress of the shaded_4096 image: interleaved pinned host
+    floats -> device planes (one PCIe copy + device deinterleave), planes ->
+    host floats, and the 8-bit egress (device quantise + 3 B/px copy back).
+    The reference's call pattern (one 2-D copy of 4-byte rows per channel,
+    samples/src/main.cu:84-114) is timed on a 256x256 crop."""
+    import ctypes
+    import torch
+    import split_amd as sa
+    from split_amd import workloads as W
+    w = W.SHADED_4096
+    n = w.n
+    img = W.make_image(w.image)
+    host = torch.from_numpy(img.T.copy()).pin_memory()          # (N, 3) interleaved
+    back = torch.empty_like(host).pin_memory()
+    q_host = torch.empty((n, 3), dtype=torch.uint8).pin_memory()
+    P = torch.empty((3, n), dtype=torch.float32, device="cuda")
+    Q = torch.empty((n, 3), dtype=torch.uint8, device="cuda")
+    temp = torch.empty(sa.lib().split_hip_interleaved_workspace(n, 3), dtype=torch.uint8,
+                       device="cuda")
+    reps = max(3, args.steps)
+    ms_up = _events_ms(lambda: sa.make_device_image(host, P, temp=temp), reps, args.warmup)
+    ok_up = bool(torch.equal(P.cpu(), host.t().contiguous()))
+    ms_down = _events_ms(lambda: sa.make_host_image(P, back, temp=temp), reps, args.warmup)
+    ok_down = bool(torch.equal(back, host))
+
+    def egress8():
+        sa.quantize(P, Q)
+        q_host.copy_(Q, non_blocking=True)
+        torch.cuda.synchronize()
+
+    ms_q = _events_ms(egress8, reps, args.warmup)
+    ms_dd = _events_ms(lambda: sa.deinterleave(temp.view(torch.float32)[:3 * n].view(n, 3), P),
+                       reps * 10, args.warmup)
+    # the reference's pattern on a 256x256 crop: per channel, one 2-D copy of
+    # npixels rows of one float with a source pitch of 3 floats
+    side = 256
+    m = side * side
+    small = host[:m].contiguous().pin_memory()
+    Ps = torch.empty((3, m), dtype=torch.float32, device="cuda")
+    # the HIP runtime torch already loaded (same file: no second runtime)
+    with open("/proc/self/maps") as f:
+        hip_path = next(ln.split()[-1] for ln in f if "libamdhip64.so" in ln)
+    hip = ctypes.CDLL(hip_path)
+    hip.hipMemcpy2D.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                ctypes.c_int]
+
+    def ref_pattern():
+        for c in range(3):
+            rc = hip.hipMemcpy2D(Ps.data_ptr() + 4 * m * c, 4, small.data_ptr() + 4 * c, 12,
+                                 4, m, 1)
+            if rc != 0:
+                raise RuntimeError(f"hipMemcpy2D failed: {rc}")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref_pattern()
+    torch.cuda.synchronize()
+    ms_ref = (time.perf_counter() - t0) * 1e3
+    ok_ref = bool(torch.equal(Ps.cpu(), small.t().contiguous()))
+    gbs = lambda b, ms: round(b / (ms / 1e3) / 1e9, 2)
+    print(json.dumps({
+        "metric": "image ingest Mpix/s (interleaved pinned host floats -> device planes)",
+        "value": round(n / (ms_up / 1e3) / 1e6, 3), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": reps, "warmup": args.warmup, "ms_per_step": round(ms_up, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": "ingest_4096", "image": "4096x4096", "channels": 3,
+                   "parallelism": "single GPU"},
+        "upload": {"ms": round(ms_up, 4), "pcie_GB_s": gbs(12 * n, ms_up), "exact": ok_up},
+        "download": {"ms": round(ms_down, 4), "pcie_GB_s": gbs(12 * n, ms_down),
+                     "exact": ok_down},
+        "egress_u8": {"ms": round(ms_q, 4), "Mpix_s": round(n / (ms_q / 1e3) / 1e6, 3)},
+        "device_deinterleave": {"ms": round(ms_dd, 4), "hbm_GB_s": gbs(24 * n, ms_dd),
+                                "hbm_frac": round(24 * n / (ms_dd / 1e3) / (HBM_PEAK_GBS * 1e9),
+                                                  4)},
+        "reference_pattern_256": {"ms": round(ms_ref, 3),
+                                  "Mpix_s": round(m / (ms_ref / 1e3) / 1e6, 3),
+                                  "ours_same_size_Mpix_s_est": round(n / (ms_up / 1e3) / 1e6, 3),
+                                  "exact": ok_ref},
+    }), flush=True)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -370,6 +518,12 @@ def main():
         return
     if args.workload == "segmeans_4096":
         segmeans_bench(args)
+        return
+    if args.workload == "color_4096":
+        color_bench(args)
+        return
+    if args.workload == "ingest_4096":
+        ingest_bench(args)
         return
     w = W.WORKLOADS[args.workload]
     if args.profile_driver:
@@ -464,7 +618,8 @@ def main():
         bytes_per_launch = 12 * n
         achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
         out["roofline"] = {
-            "kernel": "tiles_step<1, 3, false> (bounded Lloyd iteration: skip test, re-assign, exact sums, finalize)",
+            "kernel": "Lloyd iteration: tiles_step<1, 3, false> (skip test, re-assign, exact sums) "
+                      "+ tiles_reduce (totals, centroids, shift, stop flag)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

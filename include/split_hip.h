@@ -339,6 +339,95 @@ int split_hip_ws_predict(const float* params,
 int split_hip_ws_targets(const float* centroids, int32_t count, float* targets,
                          split_hip_stream_t stream);
 
+/* ---- colour features feeding k-means ------------------------------------ */
+
+/* Conversions of split/include/split/device/color/conversion.cuh:15-62
+ * (functors rgb_to_xyz, xyz_to_rgb, xyz_to_lab, lab_to_xyz, rgb_to_lab,
+ * lab_to_rgb, rgb_to_ic; formulas in detail/conversion.inl:13-139). */
+enum split_hip_color_conversion {
+  SPLIT_HIP_RGB_TO_XYZ = 0,
+  SPLIT_HIP_XYZ_TO_RGB = 1,
+  SPLIT_HIP_XYZ_TO_LAB = 2,
+  SPLIT_HIP_LAB_TO_XYZ = 3,
+  SPLIT_HIP_RGB_TO_LAB = 4,
+  SPLIT_HIP_LAB_TO_RGB = 5,
+  SPLIT_HIP_RGB_TO_IC = 6
+};
+
+/* Replaces split::device::color::convert_color_space
+ * (split/include/split/device/color/conversion.cuh:67-82): out = the
+ * conversion of each pixel of `in` (3 x n SoA planes each; out == in is
+ * allowed, any other overlap is an argument error).  sRGB companding,
+ * sRGB/XYZ matrices and CIE Lab with a D65 white (X, Y, Z scaled to 100). */
+int split_hip_convert_color_space(const float* in, float* out, int64_t n,
+                                  int32_t conversion,
+                                  split_hip_stream_t stream);
+
+/* Temp needed by split_hip_beta_feature and split_hip_lab_beta_feature
+ * (4 KiB: copies of the two maxima). */
+size_t split_hip_beta_feature_workspace(void);
+
+/* Replaces split::device::color::beta_feature
+ * (split/include/split/device/color/beta_feature.cuh:13-16,
+ *  split/src/split/device/color/beta_feature.cu:48-78).  lab: 3 x n planes
+ * (L, a, b).  With Lmax = max L and lambda = 0.2 * max |a| (the a plane
+ * only, as the reference): beta[i] = -mu if |a_i| < lambda, |b_i| < lambda
+ * and L_i < 0.2 Lmax; +mu if |a_i|, |b_i| < lambda and L_i > 0.95 Lmax;
+ * 0 otherwise.  beta may be one of lab's planes (the sample writes it over
+ * L, samples/src/main.cu:224). */
+int split_hip_beta_feature(const float* lab, int64_t n, float* beta, float mu,
+                           void* temp, size_t temp_bytes,
+                           split_hip_stream_t stream);
+
+/* The sample's feature sequence (samples/src/main.cu:216-225) in two
+ * passes: rgb -> Lab into `features` (3 x n), L copied to `luminance`
+ * (n floats, may be NULL), then beta (as split_hip_beta_feature, mu) written
+ * over the L plane.  features == rgb is allowed. */
+int split_hip_lab_beta_feature(const float* rgb, int64_t n, float* features,
+                               float* luminance, float mu, void* temp,
+                               size_t temp_bytes, split_hip_stream_t stream);
+
+/* ---- image ingest and egress -------------------------------------------
+ * Host images are pixel-interleaved (c channels per pixel, as
+ * split::host::stbi::loadf returns them, split/src/split/host/stbi/
+ * stbi_raii.cpp:115-124); the device path uses SoA planes (channel c of
+ * pixel i at planes[c*n + i]).  channels = 1..4. */
+
+/* Device temp of the host transfers: n * channels floats. */
+size_t split_hip_interleaved_workspace(int64_t n, int32_t channels);
+
+/* Replaces make_device_image (samples/src/main.cu:98-114, one cudaMemcpy2D
+ * of 4-byte rows per channel): one contiguous host-to-device copy of the
+ * interleaved image into temp, then a device deinterleave into `planes`.
+ * Host-synchronous (the host buffer may be reused on return); pinned host
+ * memory makes the copy run at full PCIe rate. */
+int split_hip_upload_interleaved(const float* host, int64_t n, int32_t channels,
+                                 float* planes, void* temp, size_t temp_bytes,
+                                 split_hip_stream_t stream);
+
+/* Replaces make_host_image (samples/src/main.cu:116-132): device interleave
+ * of `planes` into temp, one contiguous device-to-host copy.
+ * Host-synchronous (the host buffer is complete on return). */
+int split_hip_download_interleaved(const float* planes, int64_t n,
+                                   int32_t channels, float* host, void* temp,
+                                   size_t temp_bytes, split_hip_stream_t stream);
+
+/* Device-to-device forms of the two layout changes (stream-ordered):
+ * hwc[i*channels + c] <-> planes[c*n + i]. */
+int split_hip_deinterleave(const float* hwc, int64_t n, int32_t channels,
+                           float* planes, split_hip_stream_t stream);
+int split_hip_interleave(const float* planes, int64_t n, int32_t channels,
+                         float* hwc, split_hip_stream_t stream);
+
+/* The 8-bit egress of split::host::stbi::quantize
+ * (split/src/split/host/stbi/stbi_raii.cpp:135-143) fused with the
+ * interleave, on the device: out[i*channels + c] = (uint8)(clamp(v, 0, 1) *
+ * 255 + 0.5) with v = planes[c*n + i], the multiply-add rounded once.
+ * `out` is device memory of n * channels bytes; stream-ordered. */
+int split_hip_quantize_interleaved(const float* planes, int64_t n,
+                                   int32_t channels, uint8_t* out,
+                                   split_hip_stream_t stream);
+
 /* k-means inertia sum_i min_j |p_i - c_j|^2 (float distances in the
  * assignment's form, summed in double in a fixed order).  No counterpart in
  * the reference, which reports no inertia.  Host-synchronous. */

@@ -145,6 +145,23 @@ CABI = {
     "split_hip_ws_predict": (ctypes.c_int, [_c_void_p, ctypes.POINTER(ImageDesc), _c_i32,
                                             _c_void_p, _c_size_t, _c_void_p]),
     "split_hip_ws_targets": (ctypes.c_int, [_c_void_p, _c_i32, _c_void_p, _c_void_p]),
+    "split_hip_convert_color_space": (ctypes.c_int, [_c_void_p, _c_void_p, _c_i64, _c_i32,
+                                                     _c_void_p]),
+    "split_hip_beta_feature_workspace": (_c_size_t, []),
+    "split_hip_beta_feature": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p, _c_float,
+                                              _c_void_p, _c_size_t, _c_void_p]),
+    "split_hip_lab_beta_feature": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p, _c_void_p,
+                                                  _c_float, _c_void_p, _c_size_t, _c_void_p]),
+    "split_hip_interleaved_workspace": (_c_size_t, [_c_i64, _c_i32]),
+    "split_hip_upload_interleaved": (ctypes.c_int, [_c_void_p, _c_i64, _c_i32, _c_void_p,
+                                                    _c_void_p, _c_size_t, _c_void_p]),
+    "split_hip_download_interleaved": (ctypes.c_int, [_c_void_p, _c_i64, _c_i32, _c_void_p,
+                                                      _c_void_p, _c_size_t, _c_void_p]),
+    "split_hip_deinterleave": (ctypes.c_int, [_c_void_p, _c_i64, _c_i32, _c_void_p,
+                                              _c_void_p]),
+    "split_hip_interleave": (ctypes.c_int, [_c_void_p, _c_i64, _c_i32, _c_void_p, _c_void_p]),
+    "split_hip_quantize_interleaved": (ctypes.c_int, [_c_void_p, _c_i64, _c_i32, _c_void_p,
+                                                      _c_void_p]),
     "split_hip_inertia_workspace": (_c_size_t, []),
     "split_hip_inertia": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p, _c_i32, _c_void_p,
                                          _c_size_t, ctypes.POINTER(ctypes.c_double),
@@ -357,6 +374,156 @@ def compress_labels(labels, temp=None) -> int:
 
 ccl = SimpleNamespace(connected_components=connected_components,
                       compress_labels=compress_labels)
+
+
+# ----------------------------------------------------------------- color ---
+
+# split_hip_color_conversion values, named after the reference's functors
+# (split/include/split/device/color/conversion.cuh:15-62)
+CONVERSIONS = {"rgb_to_xyz": 0, "xyz_to_rgb": 1, "xyz_to_lab": 2, "lab_to_xyz": 3,
+               "rgb_to_lab": 4, "lab_to_rgb": 5, "rgb_to_ic": 6}
+
+
+def convert_color_space(points_in, points_out, converter: str) -> None:
+    """split::device::color::convert_color_space (conversion.cuh:67-82) with
+    the converter named as the reference's functor ("rgb_to_lab", ...).
+    points_out may be points_in (in place)."""
+    if converter not in CONVERSIONS:
+        raise ValueError(f"unknown converter {converter!r}; one of {sorted(CONVERSIONS)}")
+    pi, n = _points(points_in)
+    po, n2 = _points(points_out)
+    if n2 != n:
+        raise ValueError("in and out must both be (3, N)")
+    _check(lib().split_hip_convert_color_space(pi, po, n, CONVERSIONS[converter], _stream()),
+           "convert_color_space")
+
+
+def beta_feature(lab_points, beta, mu: float = 1e5, temp=None) -> None:
+    """split::device::color::beta_feature (beta_feature.cu:48-78): beta (N
+    floats, may be a row of lab_points) from the (3, N) Lab points."""
+    torch = _torch()
+    p, n = _points(lab_points)
+    b = _dev(beta, torch.float32, "beta")
+    if beta.numel() != n:
+        raise ValueError("beta must hold N entries")
+    L = lib()
+    if temp is None:
+        temp = _temp(L.split_hip_beta_feature_workspace(), lab_points)
+    _check(L.split_hip_beta_feature(p, n, b, float(mu), temp.data_ptr(), temp.numel(),
+                                    _stream()), "beta_feature")
+
+
+def lab_beta_feature(rgb_points, features, luminance=None, mu: float = 1e5,
+                     temp=None) -> None:
+    """The sample's feature sequence (main.cu:216-225): Lab of rgb_points into
+    features (3, N), L copied into luminance (N, optional), beta over L."""
+    torch = _torch()
+    p, n = _points(rgb_points)
+    f, n2 = _points(features)
+    if n2 != n:
+        raise ValueError("rgb_points and features must both be (3, N)")
+    lum = None
+    if luminance is not None:
+        lum = _dev(luminance, torch.float32, "luminance")
+        if luminance.numel() != n:
+            raise ValueError("luminance must hold N entries")
+    L = lib()
+    if temp is None:
+        temp = _temp(L.split_hip_beta_feature_workspace(), rgb_points)
+    _check(L.split_hip_lab_beta_feature(p, n, f, lum, float(mu), temp.data_ptr(),
+                                        temp.numel(), _stream()), "lab_beta_feature")
+
+
+color = SimpleNamespace(convert_color_space=convert_color_space, beta_feature=beta_feature,
+                        lab_beta_feature=lab_beta_feature, CONVERSIONS=CONVERSIONS)
+
+
+# -------------------------------------------------------------------- io ---
+
+def _hwc_host(host):
+    """(ptr, n, channels) of a C-contiguous float32 numpy array or CPU tensor
+    of shape (..., C): the last axis is the channel."""
+    import numpy as np
+    torch = _torch()
+    if isinstance(host, torch.Tensor):
+        if host.is_cuda or host.dtype != torch.float32 or not host.is_contiguous():
+            raise TypeError("host image must be a contiguous float32 CPU tensor")
+        c = host.shape[-1]
+        return host.data_ptr(), host.numel() // c, c
+    if not isinstance(host, np.ndarray) or host.dtype != np.float32 or \
+            not host.flags["C_CONTIGUOUS"]:
+        raise TypeError("host image must be a C-contiguous float32 array")
+    c = host.shape[-1]
+    return host.ctypes.data, host.size // c, c
+
+
+def make_device_image(host_hwc, planes, temp=None) -> None:
+    """make_device_image (samples/src/main.cu:98-114): interleaved host image
+    (..., C) -> device planes (C, N).  Host-synchronous."""
+    torch = _torch()
+    h, n, c = _hwc_host(host_hwc)
+    d = _dev(planes, torch.float32, "planes")
+    if planes.dim() != 2 or tuple(planes.shape) != (c, n):
+        raise ValueError(f"planes must be ({c}, {n})")
+    L = lib()
+    if temp is None:
+        temp = _temp(L.split_hip_interleaved_workspace(n, c), planes)
+    _check(L.split_hip_upload_interleaved(h, n, c, d, temp.data_ptr(), temp.numel(),
+                                          _stream()), "upload_interleaved")
+
+
+def make_host_image(planes, host_hwc, temp=None) -> None:
+    """make_host_image (samples/src/main.cu:116-132): device planes (C, N) ->
+    interleaved host image (..., C).  Host-synchronous."""
+    torch = _torch()
+    h, n, c = _hwc_host(host_hwc)
+    d = _dev(planes, torch.float32, "planes")
+    if planes.dim() != 2 or tuple(planes.shape) != (c, n):
+        raise ValueError(f"planes must be ({c}, {n})")
+    L = lib()
+    if temp is None:
+        temp = _temp(L.split_hip_interleaved_workspace(n, c), planes)
+    _check(L.split_hip_download_interleaved(d, n, c, h, temp.data_ptr(), temp.numel(),
+                                            _stream()), "download_interleaved")
+
+
+def deinterleave(hwc, planes) -> None:
+    """Device (N, C) or (..., C) interleaved -> (C, N) planes."""
+    torch = _torch()
+    c = planes.shape[0]
+    n = planes.shape[1]
+    if hwc.numel() != n * c or hwc.shape[-1] != c:
+        raise ValueError("hwc must hold N pixels of C channels")
+    _check(lib().split_hip_deinterleave(_dev(hwc, torch.float32, "hwc"), n, c,
+                                        _dev(planes, torch.float32, "planes"), _stream()),
+           "deinterleave")
+
+
+def interleave(planes, hwc) -> None:
+    """Device (C, N) planes -> (..., C) interleaved."""
+    torch = _torch()
+    c, n = planes.shape
+    if hwc.numel() != n * c or hwc.shape[-1] != c:
+        raise ValueError("hwc must hold N pixels of C channels")
+    _check(lib().split_hip_interleave(_dev(planes, torch.float32, "planes"), n, c,
+                                      _dev(hwc, torch.float32, "hwc"), _stream()),
+           "interleave")
+
+
+def quantize(planes, out) -> None:
+    """stbi::quantize (stbi_raii.cpp:135-143) of device planes (C, N) into a
+    device uint8 (..., C) interleaved image."""
+    torch = _torch()
+    c, n = planes.shape
+    if out.numel() != n * c or out.shape[-1] != c:
+        raise ValueError("out must hold N pixels of C channels")
+    _check(lib().split_hip_quantize_interleaved(_dev(planes, torch.float32, "planes"), n, c,
+                                                _dev(out, torch.uint8, "out"), _stream()),
+           "quantize_interleaved")
+
+
+io = SimpleNamespace(make_device_image=make_device_image, make_host_image=make_host_image,
+                     deinterleave=deinterleave, interleave=interleave, quantize=quantize)
 
 
 # --------------------------------------------------------------- segment ---

@@ -27,7 +27,8 @@ OBJ = LIB / "obj"
 INCLUDE = ROOT / "include"
 
 ARCH = "gfx950"
-HIP_SOURCES = ["kmeans.hip", "lloyd_tiles.hip", "ccl.hip", "batch.hip", "gen_device.hip", "warmstart.hip", "segmeans.hip", "capi.hip"]
+HIP_SOURCES = ["kmeans.hip", "lloyd_tiles.hip", "ccl.hip", "batch.hip", "gen_device.hip", "warmstart.hip",
+               "segmeans.hip", "color.hip", "io.hip", "capi.hip"]
 CXX_SOURCES = ["split_api.cpp"]
 GEN_SOURCES = ["gen.c"]
 

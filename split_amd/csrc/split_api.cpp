@@ -10,9 +10,12 @@
 #include <string>
 
 #include "split/device/ccl/compress_labels.cuh"
+#include "split/device/color/beta_feature.cuh"
+#include "split/device/color/conversion.cuh"
 #include "split/device/ccl/connected_components.cuh"
 #include "split/device/device_error.h"
 #include "split/device/kmeans/kmeans.cuh"
+#include "split/host/image_transfer.h"
 #include "split_hip.h"
 
 namespace {
@@ -38,6 +41,25 @@ int rows3(std::int64_t rows, const char* what) {
                               std::string(what) + ": points must have 3 rows");
   return 3;
 }
+
+// Stream-ordered device scratch for the wrappers whose reference
+// signatures take no temp argument; freed when the wrapper returns.
+class Scratch {
+ public:
+  Scratch(std::size_t bytes, const char* what) {
+    const hipError_t e = hipMallocAsync(&p_, bytes, nullptr);
+    if (e != hipSuccess)
+      throw split::device_error(SPLIT_HIP_ERR_HIP,
+                                std::string(what) + ": " + hipGetErrorString(e));
+  }
+  ~Scratch() { (void)hipFreeAsync(p_, nullptr); }
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+  void* get() const { return p_; }
+
+ private:
+  void* p_ = nullptr;
+};
 
 }  // namespace
 
@@ -214,4 +236,63 @@ int compress_labels(view1d<int> dio_labels, device_ptr<void> dio_temp)
 
 }  // namespace ccl
 
+namespace color
+{
+// split/include/split/device/color/conversion.cuh:67-82
+void convert_color_space(const_view2d<real> di_in, view2d<real> do_out,
+                         int conversion)
+{
+  rows3(di_in.num_rows, "convert_color_space");
+  rows3(do_out.num_rows, "convert_color_space");
+  if (do_out.num_cols != di_in.num_cols)
+    throw split::device_error(SPLIT_HIP_ERR_ARGUMENT,
+                              "convert_color_space: in and out sizes differ");
+  check(split_hip_convert_color_space(di_in.values.data(), do_out.values.data(),
+                                      di_in.num_cols, conversion, nullptr),
+        "convert_color_space");
+  sync("convert_color_space");
+}
+
+// split/src/split/device/color/beta_feature.cu:48-78
+void beta_feature(const_view2d<real> di_lab_points, view1d<real> do_beta,
+                  const real mu)
+{
+  rows3(di_lab_points.num_rows, "beta_feature");
+  const std::size_t bytes = split_hip_beta_feature_workspace();
+  Scratch temp(bytes, "beta_feature");
+  check(split_hip_beta_feature(di_lab_points.values.data(), di_lab_points.num_cols,
+                               do_beta.data(), mu, temp.get(), bytes, nullptr),
+        "beta_feature");
+  sync("beta_feature");
+}
+}  // namespace color
+
 SPLIT_DEVICE_NAMESPACE_END
+
+SPLIT_HOST_NAMESPACE_BEGIN
+
+// samples/src/main.cu:98-114
+void make_device_image(const real* h_image, split::device::view2d<real> d_image)
+{
+  const std::int64_t n = d_image.num_cols;
+  const int c = (int)d_image.num_rows;
+  const std::size_t bytes = split_hip_interleaved_workspace(n, c);
+  Scratch temp(std::max<std::size_t>(bytes, 1), "make_device_image");
+  check(split_hip_upload_interleaved(h_image, n, c, d_image.values.data(), temp.get(),
+                                     bytes, nullptr),
+        "make_device_image");
+}
+
+// samples/src/main.cu:116-132
+void make_host_image(split::device::view2d<real> d_image, real* h_image)
+{
+  const std::int64_t n = d_image.num_cols;
+  const int c = (int)d_image.num_rows;
+  const std::size_t bytes = split_hip_interleaved_workspace(n, c);
+  Scratch temp(std::max<std::size_t>(bytes, 1), "make_host_image");
+  check(split_hip_download_interleaved(d_image.values.data(), n, c, h_image,
+                                       temp.get(), bytes, nullptr),
+        "make_host_image");
+}
+
+SPLIT_HOST_NAMESPACE_END

@@ -74,6 +74,9 @@ def test_cpp_dropin_headers_compile(tmp_path):
 #include "split/device/kmeans/kmeans.cuh"
 #include "split/device/ccl/connected_components.cuh"
 #include "split/device/ccl/compress_labels.cuh"
+#include "split/device/color/conversion.cuh"
+#include "split/device/color/beta_feature.cuh"
+#include "split/host/image_transfer.h"
 #include "split/device/device_error.h"
 using namespace split::device;
 int main() {
@@ -91,6 +94,12 @@ int main() {
     ccl::connected_components(const_view2d<int>(64, 64, l), temp, view1d<int>(s, n), 100);
     (void)ccl::compress_labels(view1d<int>(s, n), temp);
     kmeans::calculate_centroids(const_view1d<int>(labels), pts, cent, temp);
+    view2d<split::real> lab(3, n, p);
+    color::convert_color_space(pts, lab, color::rgb_to_lab<>());
+    color::convert_color_space(lab, lab, color::lab_to_rgb<>{});
+    color::beta_feature(lab, lab.row(0));
+    split::host::make_device_image(p, lab);
+    split::host::make_host_image(lab, p);
   }
   return 0;
 }
