@@ -22,7 +22,8 @@ size and k, one graph-replayed launch sequence) against the same images
 through 256 separate segment calls.  Other workloads: cc_stress_8192 (CC +
 compress of the spiral/checkerboard labels), segmeans_4096 (segment means +
 propagation), color_4096 (Lab + beta features), ingest_4096 (host image
-upload / download / 8-bit egress), and the image workloads of
+upload / download / 8-bit egress), erode_4096 (erosion of the segment
+labels), and the image workloads of
 split_amd.workloads (plain_512, stress_8192, ...).
 """
 from __future__ import annotations
@@ -381,6 +382,50 @@ def color_bench(args):
     }), flush=True)
 
 
+def erode_bench(args):
+    """Erosion (n = 10, the sample's call at samples/src/main.cu:302) of the
+    shaded_4096 segment labels: the column sweep then the row sweep."""
+    import torch
+    import split_amd as sa
+    from split_amd import workloads as W
+    w = W.SHADED_4096
+    P = torch.from_numpy(W.make_image(w.image)).cuda()
+    C = torch.empty((3, w.k), dtype=torch.float32, device="cuda")
+    L = torch.empty(w.n, dtype=torch.int32, device="cuda")
+    S = torch.empty(w.n, dtype=torch.int32, device="cuda")
+    sa.uniform_random_initialize(P, C, w.init_seed)
+    r = sa.segment(P, 4096, 4096, C, L, S, max_iter=w.max_iter, threshold=w.threshold,
+                   connectivity=w.connectivity)
+    E = torch.empty_like(S).view(4096, 4096)
+    temp = torch.empty(sa.lib().split_hip_erode_workspace(4096, 4096), dtype=torch.uint8,
+                       device="cuda")
+    stream = torch.cuda.current_stream()
+    total, reps = 0.0, args.steps * 10
+    for i in range(args.warmup + reps):
+        E.view(-1).copy_(S)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        sa.erode(E, 10, temp=temp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if i >= args.warmup:
+            total += e0.elapsed_time(e1)
+    ms = total / reps
+    kept = int((E >= 0).sum().item())
+    print(json.dumps({
+        "metric": "erosion Mpix/s (n = 10, segment labels)",
+        "value": round(w.n / (ms / 1e3) / 1e6, 3), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": reps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": "erode_4096", "image": "4096x4096", "niterations": 10,
+                   "segments": r.num_segments, "pixels_kept": kept,
+                   "parallelism": "single GPU"},
+        "algorithmic_bytes_per_step": 8 * w.n,
+        "hbm_frac": round(8 * w.n / (ms / 1e3) / (HBM_PEAK_GBS * 1e9), 4),
+    }), flush=True)
+
+
 def ingest_bench(args):
     """Host ingest / egress of the shaded_4096 image: interleaved pinned host
     floats -> device planes (one PCIe copy + device deinterleave), planes ->
@@ -524,6 +569,9 @@ def main():
         return
     if args.workload == "ingest_4096":
         ingest_bench(args)
+        return
+    if args.workload == "erode_4096":
+        erode_bench(args)
         return
     w = W.WORKLOADS[args.workload]
     if args.profile_driver:

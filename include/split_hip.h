@@ -339,6 +339,36 @@ int split_hip_ws_predict(const float* params,
 int split_hip_ws_targets(const float* centroids, int32_t count, float* targets,
                          split_hip_stream_t stream);
 
+/* ---- erosion of segment labels ------------------------------------------ */
+
+/* Temp needed by split_hip_erode and split_hip_erode_horizontal_sweep:
+ * width * height int32. */
+size_t split_hip_erode_workspace(int32_t width, int32_t height);
+
+/* Replaces split::device::morph::erode_horizontal_sweep
+ * (split/include/split/device/morph/erode.cuh:12-14,
+ *  split/src/split/device/morph/erode.cu:57-139): in each row of the
+ * height x width labels, a run [s, e) of equal labels keeps the pixels
+ * s + n <= p < e - n (n = niterations) and every other pixel becomes -1;
+ * i.e. a pixel survives iff the 2n + 1 pixels centred on it in its row lie
+ * inside the row and carry its label.  In place; niterations = 0 leaves the
+ * labels unchanged (the reference's 0 writes past its buffer and drops every
+ * other run). */
+int split_hip_erode_horizontal_sweep(int32_t* labels, int32_t width,
+                                     int32_t height, int32_t niterations,
+                                     void* temp, size_t temp_bytes,
+                                     split_hip_stream_t stream);
+
+/* Replaces split::device::morph::erode
+ * (split/include/split/device/morph/erode.cuh:16-18,
+ *  split/src/split/device/morph/erode.cu:141-159): the sweep over the
+ * columns, then over the rows of the result.  Together: a pixel keeps its
+ * label iff the (2n + 1) x (2n + 1) square centred on it lies inside the
+ * image and carries one label; every other pixel becomes -1.  In place. */
+int split_hip_erode(int32_t* labels, int32_t width, int32_t height,
+                    int32_t niterations, void* temp, size_t temp_bytes,
+                    split_hip_stream_t stream);
+
 /* ---- colour features feeding k-means ------------------------------------ */
 
 /* Conversions of split/include/split/device/color/conversion.cuh:15-62

@@ -162,6 +162,11 @@ CABI = {
     "split_hip_interleave": (ctypes.c_int, [_c_void_p, _c_i64, _c_i32, _c_void_p, _c_void_p]),
     "split_hip_quantize_interleaved": (ctypes.c_int, [_c_void_p, _c_i64, _c_i32, _c_void_p,
                                                       _c_void_p]),
+    "split_hip_erode_workspace": (_c_size_t, [_c_i32, _c_i32]),
+    "split_hip_erode_horizontal_sweep": (ctypes.c_int, [_c_void_p, _c_i32, _c_i32, _c_i32,
+                                                        _c_void_p, _c_size_t, _c_void_p]),
+    "split_hip_erode": (ctypes.c_int, [_c_void_p, _c_i32, _c_i32, _c_i32, _c_void_p,
+                                       _c_size_t, _c_void_p]),
     "split_hip_inertia_workspace": (_c_size_t, []),
     "split_hip_inertia": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p, _c_i32, _c_void_p,
                                          _c_size_t, ctypes.POINTER(ctypes.c_double),
@@ -374,6 +379,37 @@ def compress_labels(labels, temp=None) -> int:
 
 ccl = SimpleNamespace(connected_components=connected_components,
                       compress_labels=compress_labels)
+
+
+# ----------------------------------------------------------------- morph ---
+
+def _erode_call(fn: str, labels_hw, niterations: int, temp):
+    torch = _torch()
+    if labels_hw.dim() != 2:
+        raise ValueError("labels must be (H, W)")
+    h, w = labels_hw.shape
+    lb = _dev(labels_hw, torch.int32, "labels")
+    L = lib()
+    if temp is None:
+        temp = _temp(L.split_hip_erode_workspace(w, h), labels_hw)
+    _check(getattr(L, fn)(lb, w, h, int(niterations), temp.data_ptr(), temp.numel(),
+                          _stream()), fn[len("split_hip_"):])
+
+
+def erode(labels_hw, niterations: int = 1, temp=None) -> None:
+    """split::device::morph::erode (erode.cu:141-159), in place: a pixel
+    keeps its label iff the (2n+1)^2 square around it lies in the image and
+    carries one label; other pixels become -1."""
+    _erode_call("split_hip_erode", labels_hw, niterations, temp)
+
+
+def erode_horizontal_sweep(labels_hw, niterations: int = 1, temp=None) -> None:
+    """split::device::morph::erode_horizontal_sweep (erode.cu:57-139), in
+    place: the row sweep alone."""
+    _erode_call("split_hip_erode_horizontal_sweep", labels_hw, niterations, temp)
+
+
+morph = SimpleNamespace(erode=erode, erode_horizontal_sweep=erode_horizontal_sweep)
 
 
 # ----------------------------------------------------------------- color ---

@@ -28,7 +28,7 @@ INCLUDE = ROOT / "include"
 
 ARCH = "gfx950"
 HIP_SOURCES = ["kmeans.hip", "lloyd_tiles.hip", "ccl.hip", "batch.hip", "gen_device.hip", "warmstart.hip",
-               "segmeans.hip", "color.hip", "io.hip", "capi.hip"]
+               "segmeans.hip", "color.hip", "io.hip", "erode.hip", "capi.hip"]
 CXX_SOURCES = ["split_api.cpp"]
 GEN_SOURCES = ["gen.c"]
 

@@ -142,25 +142,33 @@ __global__ void batch_init(BatchArgs a) {
   for (int i = threadIdx.x; i < k; i += blockDim.x) m.fcum[i] = 0.f;
 }
 
-// Largest |v| per image, one wave per Lloyd tile.
+// Largest |v| per image.  A wave takes a contiguous run of the batch's
+// Lloyd tiles and keeps a running maximum while the run stays in one image,
+// so an image receives one atomic per wave that touches it.
 __global__ __launch_bounds__(256) void batch_absmax(BatchArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
-  for (int64_t s = gw; s < a.tiles; s += nw) {
+  const int64_t s1 = a.tiles * (gw + 1) / nw;
+  int64_t s = a.tiles * gw / nw;
+  while (s < s1) {
     const int i = find_image<&BatchImg::tile0>(a.img, a.count, s);
     const BatchImg& m = a.img[i];
     const Geometry g = image_geometry(m);
-    int64_t idx[4];
-    bool v[4], lvec;
-    segment_index<kTileVec>(g, s - m.tile0, lane, idx, v, lvec);
-    lvec = lvec && (g.width % 4 == 0) && ((((uintptr_t)g.pts) & 15u) == 0);
-    float x[4], y[4], z[4];
-    load_colours(g, idx, v, lvec, x, y, z);
+    const int64_t end = std::min<int64_t>(s1, m.tile0 + g.nseg);
+    const bool vec = (g.width % 4 == 0) && ((((uintptr_t)g.pts) & 15u) == 0);
     float mx = 0.f;
+    for (; s < end; ++s) {
+      int64_t idx[4];
+      bool v[4], lvec;
+      segment_index<kTileVec>(g, s - m.tile0, lane, idx, v, lvec);
+      lvec = lvec && vec;
+      float x[4], y[4], z[4];
+      load_colours(g, idx, v, lvec, x, y, z);
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      mx = v[j] ? fmaxf(mx, fmaxf(fabsf(x[j]), fmaxf(fabsf(y[j]), fabsf(z[j])))) : mx;
+      for (int j = 0; j < 4; ++j)
+        mx = v[j] ? fmaxf(mx, fmaxf(fabsf(x[j]), fmaxf(fabsf(y[j]), fabsf(z[j])))) : mx;
+    }
     mx = wave_allreduce(mx, OpMax());
     if (lane == 0) atomicMax(&a.img[i].maxabs, __float_as_uint(mx));
   }

@@ -72,21 +72,16 @@ __device__ __forceinline__ float bound_step(float f, double d2j, int j,
 // per-pixel LDS atomics; more go through add_by_label's wave sums.
 constexpr int kAtomicChanges = 16;
 
-// Re-assigns up to 256 pixels (4 per lane: idx/v) whose previous labels are
-// old[] (-1: none), writes their new state, and accumulates: every pixel
-// (FULL) or the +/- changes of label (bounded iterations).
-template <int CPL, bool FULL>
-__device__ __forceinline__ void reassign(
-    const TileArgs& a, const float (&cx)[CPL], const float (&cy)[CPL],
-    const float (&cz)[CPL], const float (&cn)[CPL], const float* fl,
-    float fscale, int lane, const int64_t (&idx)[4], const bool (&v)[4],
-    bool lvec, const int (&old)[4], long long* slice, IRun& run) {
+// The second half of reassign: given the pixels' colours, new labels and
+// slack, writes their state and accumulates.
+template <bool FULL>
+__device__ __forceinline__ void commit_assignment(
+    const TileArgs& a, const float* fl, float fscale, int lane,
+    const int64_t (&idx)[4], const bool (&v)[4], bool lvec,
+    const int (&old)[4], const float (&x)[4], const float (&y)[4],
+    const float (&z)[4], const int (&lab)[4], const float (&slack)[4],
+    long long* slice, IRun& run) {
   const int k = a.k;
-  float x[4], y[4], z[4];
-  load_colours(a, idx, v, lvec, x, y, z);
-  int lab[4];
-  float slack[4];
-  assign_segment<CPL, true>(cx, cy, cz, cn, k, lane, x, y, z, v, lab, slack);
   uint32_t ns[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -163,6 +158,24 @@ __device__ __forceinline__ void reassign(
       add_by_label(slice, k, lane, from, qx, qy, qz, -1);
     }
   }
+}
+
+// Re-assigns up to 256 pixels (4 per lane: idx/v) whose previous labels are
+// old[] (-1: none), writes their new state, and accumulates: every pixel
+// (FULL) or the +/- changes of label (bounded iterations).
+template <int CPL, bool FULL>
+__device__ __forceinline__ void reassign(
+    const TileArgs& a, const float (&cx)[CPL], const float (&cy)[CPL],
+    const float (&cz)[CPL], const float (&cn)[CPL], const float* fl,
+    float fscale, int lane, const int64_t (&idx)[4], const bool (&v)[4],
+    bool lvec, const int (&old)[4], long long* slice, IRun& run) {
+  float x[4], y[4], z[4];
+  load_colours(a, idx, v, lvec, x, y, z);
+  int lab[4];
+  float slack[4];
+  assign_segment<CPL, true>(cx, cy, cz, cn, a.k, lane, x, y, z, v, lab, slack);
+  commit_assignment<FULL>(a, fl, fscale, lane, idx, v, lvec, old, x, y, z,
+                          lab, slack, slice, run);
 }
 
 // Tile source for walk_tiles: the tiles [s, end) in order.

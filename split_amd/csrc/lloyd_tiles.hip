@@ -163,8 +163,9 @@ struct BoundedWalker {
   IRun run;
   int qlen;            // wave-uniform
 
-  __device__ void tile_pixels(const TileGeo& g, int64_t (&idx)[4],
-                              bool (&v)[4], bool& lvec) const {
+  __device__ __forceinline__ void tile_pixels(const TileGeo& g,
+                                              int64_t (&idx)[4], bool (&v)[4],
+                                              bool& lvec) const {
     const int col = g.col0 + 4 * (lane & 3);
     const bool rowok = g.row0 + (lane >> 2) < a.height;
     const int o = g.off + lane_off;
@@ -176,8 +177,19 @@ struct BoundedWalker {
     lvec = MODE == kTileVec && rowok && col + 3 < a.width;
   }
 
-  __device__ void load_state(const TileGeo& g, bool have,
-                             uint32_t (&ps)[4]) const {
+  // Tiles wholly inside the image (wave-uniform: the geometry is scalar).
+  __device__ __forceinline__ bool inner(const TileGeo& g) const {
+    return g.row0 + kTileSide <= a.height && g.col0 + kTileSide <= a.width;
+  }
+
+  __device__ __forceinline__ void load_state(const TileGeo& g, bool have,
+                                             uint32_t (&ps)[4]) const {
+    if (MODE == kTileVec && have && inner(g)) {
+      const uint4 q =
+          *reinterpret_cast<const uint4*>(a.pstate + g.off + lane_off);
+      ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
+      return;
+    }
     const int col = g.col0 + 4 * (lane & 3);
     const bool rowok = have && g.row0 + (lane >> 2) < a.height;
     const uint32_t* p = a.pstate + (have ? g.off + lane_off : 0);
@@ -193,7 +205,7 @@ struct BoundedWalker {
 
   // Re-assigns the first min(qlen, 256) queued pixels, then moves the rest
   // to the front.
-  __device__ void drain() {
+  __device__ __forceinline__ void drain() {
     const int c = qlen < 256 ? qlen : 256;
     int64_t idx[4];
     bool v[4];
@@ -219,22 +231,31 @@ struct BoundedWalker {
   }
 
   // Skip test of one tile; re-assigns it whole or queues its failing pixels.
-  __device__ void visit(const TileGeo& g, const uint32_t (&ps)[4]) {
+  __device__ __forceinline__ void visit(const TileGeo& g,
+                                        const uint32_t (&ps)[4]) {
     const int col = g.col0 + 4 * (lane & 3);
     const bool rowok = g.row0 + (lane >> 2) < a.height;
-    const bool interior = g.row0 + kTileSide <= a.height &&
-                          g.col0 + kTileSide <= a.width;  // wave-uniform
     int old[4];
     bool fail[4];
     uint64_t fb[4];
     int nfail = 0;
+    if (inner(g)) {  // every pixel exists: no per-lane guards
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const bool v = interior || (rowok && col + i < a.width);
-      old[i] = (int)(ps[i] & 0xFFu);
-      fail[i] = v && !(state_budget(ps[i]) - fl[old[i]] > margin);
-      fb[i] = ballot(fail[i]);
-      nfail += __builtin_popcountll(fb[i]);
+      for (int i = 0; i < 4; ++i) {
+        old[i] = (int)(ps[i] & 0xFFu);
+        fail[i] = !(state_budget(ps[i]) - fl[old[i]] > margin);
+        fb[i] = ballot(fail[i]);
+        nfail += __builtin_popcountll(fb[i]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const bool v = rowok && col + i < a.width;
+        old[i] = (int)(ps[i] & 0xFFu);
+        fail[i] = v && !(state_budget(ps[i]) - fl[v ? old[i] : 0] > margin);
+        fb[i] = ballot(fail[i]);
+        nfail += __builtin_popcountll(fb[i]);
+      }
     }
     if (nfail == 0) return;  // every pixel provably unchanged
     if (nfail > kWholeTileFails) {
@@ -265,7 +286,8 @@ struct BoundedWalker {
 
   // Walks the units list[0, cnt), taken through *counter (zero on entry),
   // loading the state of the next tile while the current one is tested.
-  __device__ void walk(const int2* list, int cnt, unsigned* counter) {
+  __device__ __forceinline__ void walk(const int2* list, int cnt,
+                                       unsigned* counter) {
     int pend_off = 0, pend_row0 = 0, pend_col0 = 0;
     bool pending = false;
     auto next = [&](TileGeo& g) {
@@ -279,12 +301,14 @@ struct BoundedWalker {
       const int u = readfirstlane_i((int)i);
       if (u >= cnt) return false;
       const int2 e = list[u];
-      const int col0 = e.x - e.y * a.width;
-      g = TileGeo{e.x, e.y, col0};
+      // the entry is the same for every lane: keep the geometry scalar
+      const int off = readfirstlane_i(e.x), row0 = readfirstlane_i(e.y);
+      const int col0 = off - row0 * a.width;
+      g = TileGeo{off, row0, col0};
       if (col0 + kTileSide < a.width) {
         pending = true;
-        pend_off = e.x + kTileSide;
-        pend_row0 = e.y;
+        pend_off = off + kTileSide;
+        pend_row0 = row0;
         pend_col0 = col0 + kTileSide;
       }
       return true;
@@ -310,7 +334,8 @@ struct BoundedWalker {
   // re-assigned as one list, their concatenation in wave order, in batches
   // of 256 handed out through an LDS counter.  queues = the first wave's
   // queue (each wave's spans 2 * kQueueCap ints: indices, then labels).
-  __device__ void finish_block(int* queues, int wave, int nwaves) {
+  __device__ __forceinline__ void finish_block(int* queues, int wave,
+                                               int nwaves) {
     __shared__ int start[kMaxQueueWaves + 1];
     __shared__ unsigned next_batch;
     if (lane == 0) start[wave + 1] = qlen;
@@ -518,8 +543,15 @@ __global__ __launch_bounds__(256) void tiles_absmax(const float* pts,
   for (int64_t i = 4 * chunks + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
        i < count; i += stride)
     m = fmaxf(m, fabsf(pts[i]));
+  // one atomic per block (many blocks updating one address serialise)
+  __shared__ float part[256 / kWave];
   m = wave_allreduce(m, OpMax());
-  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(out, __float_as_uint(m));
+  if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)blockDim.x / kWave; ++w) m = fmaxf(m, part[w]);
+    atomicMax(out, __float_as_uint(m));
+  }
 }
 
 __global__ void tiles_labels(const LloydState* st, const uint32_t* pstate,

@@ -15,6 +15,7 @@
 #include "split/device/ccl/connected_components.cuh"
 #include "split/device/device_error.h"
 #include "split/device/kmeans/kmeans.cuh"
+#include "split/device/morph/erode.cuh"
 #include "split/host/image_transfer.h"
 #include "split_hip.h"
 
@@ -235,6 +236,38 @@ int compress_labels(view1d<int> dio_labels, device_ptr<void> dio_temp)
 }
 
 }  // namespace ccl
+
+namespace morph
+{
+namespace
+{
+void erode_call(view2d<int> labels, int n, bool columns_too, const char* what)
+{
+  const int32_t w = (int32_t)labels.num_cols, h = (int32_t)labels.num_rows;
+  const std::size_t bytes = split_hip_erode_workspace(w, h);
+  Scratch temp(std::max<std::size_t>(bytes, 1), what);
+  check(columns_too
+            ? split_hip_erode(labels.values.data(), w, h, n, temp.get(), bytes,
+                              nullptr)
+            : split_hip_erode_horizontal_sweep(labels.values.data(), w, h, n,
+                                               temp.get(), bytes, nullptr),
+        what);
+  sync(what);
+}
+}  // namespace
+
+// split/src/split/device/morph/erode.cu:57-139
+void erode_horizontal_sweep(view2d<int> dio_labels, const int niterations)
+{
+  erode_call(dio_labels, niterations, false, "erode_horizontal_sweep");
+}
+
+// split/src/split/device/morph/erode.cu:141-159
+void erode(view2d<int> dio_labels, const int niterations)
+{
+  erode_call(dio_labels, niterations, true, "erode");
+}
+}  // namespace morph
 
 namespace color
 {

@@ -77,6 +77,7 @@ def test_cpp_dropin_headers_compile(tmp_path):
 #include "split/device/color/conversion.cuh"
 #include "split/device/color/beta_feature.cuh"
 #include "split/host/image_transfer.h"
+#include "split/device/morph/erode.cuh"
 #include "split/device/device_error.h"
 using namespace split::device;
 int main() {
@@ -100,6 +101,8 @@ int main() {
     color::beta_feature(lab, lab.row(0));
     split::host::make_device_image(p, lab);
     split::host::make_host_image(lab, p);
+    morph::erode(view2d<int>(64, 64, s), 10);
+    morph::erode_horizontal_sweep(view2d<int>(64, 64, s));
   }
   return 0;
 }
