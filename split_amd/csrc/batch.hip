@@ -2,12 +2,13 @@
 // one captured hipGraph of persistent launches.
 //
 // The images' 16x16 Lloyd tiles are numbered consecutively (image i owns
-// global tiles [tile0_i, tile0_{i+1})).  Images are split into two groups,
-// each walked by its own persistent kernel once per Lloyd iteration:
-//  * plain images (k < 32): every iteration assigns every pixel of every
-//    tile (box-pruned exact assignment) and accumulates; nothing is stored
-//    per pixel, the labels come from one assignment pass at the end;
-//  * bounded images (k >= 32): the single-image bounded iteration
+// global tiles [tile0_i, tile0_{i+1})).  Images are split into two groups
+// by k (kBoundedMinK; by default every image is bounded), each walked by its
+// own persistent kernel once per Lloyd iteration:
+//  * plain images: every iteration assigns every pixel of every tile
+//    (box-pruned exact assignment) and accumulates; nothing is stored per
+//    pixel, the labels come from one assignment pass at the end;
+//  * bounded images: the single-image bounded iteration
 //    (lloyd_bounded.hpp) — the first iteration records each pixel's label
 //    and slack, later ones skip the pixels whose bound proves the label
 //    unchanged and re-assign the rest; the labels come from that state.
@@ -93,10 +94,13 @@ struct Group {
   int64_t t0, t1;
 };
 
-// Bounded iterations pay for reading and writing the per-pixel state and
-// for queueing; they win once the full assignment is expensive (k >= 32 on
-// the shaded workloads).
-constexpr int kBoundedMinK = 32;
+// Images with k >= kBoundedMinK take bounded iterations.  They pay for
+// reading and writing the per-pixel state and for queueing, and win at every
+// k on the shaded workloads (batch_256: 253 ms with every image bounded,
+// 270 / 276 / 287 ms with plain iterations below k = 8 / 16 / 32; the
+// results are identical), so every image is bounded; the plain group stays
+// for a threshold above 1.
+constexpr int kBoundedMinK = 1;
 
 // Image owning global index g of the field at byte offset OFF (tile0,
 // ctile0 or px0): the last image whose start is <= g.

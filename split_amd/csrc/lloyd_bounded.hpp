@@ -245,7 +245,22 @@ __device__ __forceinline__ void walk_tiles(
                           old, slice, run);
     }
   } else {
+    // Tile s wholly inside the image (wave-uniform); o = offset of this
+    // lane's first pixel (tile indices of one image fit in int).
+    auto tile_inner = [&](int64_t s, int64_t& o) {
+      const int t = (int)s;
+      const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
+      o = (int64_t)(ty * kTileSide + (lane >> 2)) * a.width + tx * kTileSide +
+          4 * (lane & 3);
+      return (ty + 1) * kTileSide <= a.height && (tx + 1) * kTileSide <= a.width;
+    };
     auto load_state = [&](int64_t s, bool have, uint32_t (&ps)[4]) {
+      int64_t o;
+      if (MODE == kTileVec && have && tile_inner(s, o)) {
+        const uint4 q = *reinterpret_cast<const uint4*>(a.pstate + o);
+        ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
+        return;
+      }
       int64_t idx[4];
       bool v[4], lvec;
       segment_index<MODE>(a, have ? s : 0, lane, idx, v, lvec);
@@ -260,18 +275,32 @@ __device__ __forceinline__ void walk_tiles(
     auto visit = [&](int64_t s, const uint32_t (&ps)[4]) {
       int64_t idx[4];
       bool v[4], lvec;
-      segment_index<MODE>(a, s, lane, idx, v, lvec);
       int old[4];
       bool fail[4];
       uint64_t fb[4];
       int nfail = 0;
+      int64_t o;
+      if (tile_inner(s, o)) {  // every pixel exists: no per-lane guards
+        lvec = MODE == kTileVec;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        old[i] = v[i] ? (int)(ps[i] & 0xFFu) : -1;
-        fail[i] = v[i] &&
-                  !(state_budget(ps[i]) - fl[v[i] ? old[i] : 0] > margin);
-        fb[i] = ballot(fail[i]);
-        nfail += __builtin_popcountll(fb[i]);
+        for (int i = 0; i < 4; ++i) {
+          idx[i] = o + i;
+          v[i] = true;
+          old[i] = (int)(ps[i] & 0xFFu);
+          fail[i] = !(state_budget(ps[i]) - fl[old[i]] > margin);
+          fb[i] = ballot(fail[i]);
+          nfail += __builtin_popcountll(fb[i]);
+        }
+      } else {
+        segment_index<MODE>(a, s, lane, idx, v, lvec);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          old[i] = v[i] ? (int)(ps[i] & 0xFFu) : -1;
+          fail[i] = v[i] &&
+                    !(state_budget(ps[i]) - fl[v[i] ? old[i] : 0] > margin);
+          fb[i] = ballot(fail[i]);
+          nfail += __builtin_popcountll(fb[i]);
+        }
       }
       if (nfail == 0) return;  // every pixel provably unchanged
       if (nfail > kWholeTileFails) {
