@@ -1,8 +1,9 @@
 // Bounded Lloyd iterations over 16x16 tiles with exact integer sums.
 //
-// Used by cluster_image / segment when k <= 256 (any iteration count).  Results equal plain Lloyd:
-// every label is the dense float argmin (ties to the lowest index), and the
-// centroids are exact functions of cluster membership.
+// Used by cluster_image / segment when k <= 256 (any iteration count).
+// Results equal plain Lloyd: every label is the dense float argmin (ties to
+// the lowest index), and the centroids are exact functions of cluster
+// membership.
 //
 // Per-pixel state (4 bytes, in temp): the label and a "budget" (see
 // pack_state in lloyd_bounded.hpp): a pixel provably keeps its label while
