@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void batch_absmax(BatchArgs a) {
     const BatchImg& m = a.img[i];
     const Geometry g = image_geometry(m);
     const int64_t end = std::min<int64_t>(s1, m.tile0 + g.nseg);
-    const bool vec = (g.width % 4 == 0) && ((((uintptr_t)g.pts) & 15u) == 0);
+    const bool vec = aligned4(g.pts);  // dword-aligned 16-byte loads
     float mx = 0.f;
     for (; s < end; ++s) {
       int64_t idx[4];
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(1024) void batch_step_plain(BatchArgs a, Group g) {
       cn[0] = fmaf(cz[0], cz[0], fmaf(cy[0], cy[0], cx[0] * cx[0]));
     }
     const float fscale = scale_of(m.maxabs);
-    const bool vec = (geo.width % 4 == 0) && ((((uintptr_t)geo.pts) & 15u) == 0);
+    const bool vec = aligned4(geo.pts);  // dword-aligned 16-byte loads
     IRun run{0, 0, 0, 0, -1, 0};
     for (int64_t t = s; t < end; ++t) {
       int64_t idx[4];
@@ -350,24 +350,13 @@ __global__ __launch_bounds__(1024) void batch_step_plain(BatchArgs a, Group g) {
   }
 }
 
-// One Lloyd iteration of the bounded images.  FIRST: assign every pixel and
-// record its state; otherwise the bounded iteration.
+// Tiles per chunk of the bounded iterations' round-robin deal.
+constexpr int64_t kBalanceTiles = 64;
+
+// The tiles [s, s1) of a group (possibly spanning images) for one wave.
 template <bool FIRST>
-__global__ __launch_bounds__(512) void batch_step_bounded(BatchArgs a, Group g) {
-  extern __shared__ WaveLds wlds[];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const int nwaves = blockDim.x / kWave;
-  WaveLds& w = wlds[wave];
-  for (int v = lane; v < 4 * kBatchMaxK; v += kWave) w.slice[v] = 0;
-
-  const int64_t gw = (int64_t)blockIdx.x * nwaves + wave;
-  const int64_t W = (int64_t)gridDim.x * nwaves;
-  const int64_t n = g.t1 - g.t0;
-  const int64_t s0 = g.t0 + n * gw / W;
-  const int64_t s1 = g.t0 + n * (gw + 1) / W;
-
-  int64_t s = s0;
+__device__ __forceinline__ void bounded_run(const BatchArgs& a, const Group& g, WaveLds& w,
+                            int lane, int64_t s, int64_t s1) {
   while (s < s1) {
     const int i = find_image<&BatchImg::tile0>(g.img, g.count, s);
     BatchImg& m = g.img[i];
@@ -395,8 +384,8 @@ __global__ __launch_bounds__(512) void batch_step_bounded(BatchArgs a, Group g) 
     const float margin = 1e-5f * (1.f + 4.f * maxabs);
     if (lane < k) w.fl[lane] = m.fcum[lane];
     __builtin_amdgcn_wave_barrier();
-    const bool vec = (ta.width % 4 == 0) && ((((uintptr_t)ta.pts) & 15u) == 0) &&
-                     ((((uintptr_t)ta.pstate) & 15u) == 0);
+    // dword-aligned 16-byte loads and stores: any width
+    const bool vec = aligned4(ta.pts) && aligned4(ta.pstate);
     const int64_t t0 = s - m.tile0, t1 = end - m.tile0;
     if (vec)
       walk_tiles<1, kTileVec, FIRST>(ta, TileRange{t0, t1}, cx, cy, cz,
@@ -408,6 +397,34 @@ __global__ __launch_bounds__(512) void batch_step_bounded(BatchArgs a, Group g) 
                                   w.qold, lane);
     batch_finish_segment(a, m, (int)(end - s), w, lane);
     s = end;
+  }
+}
+
+// One Lloyd iteration of the bounded images.  FIRST: assign every pixel and
+// record its state; otherwise the bounded iteration.
+template <bool FIRST>
+__global__ __launch_bounds__(512) void batch_step_bounded(BatchArgs a, Group g) {
+  extern __shared__ WaveLds wlds[];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int nwaves = blockDim.x / kWave;
+  WaveLds& w = wlds[wave];
+  for (int v = lane; v < 4 * kBatchMaxK; v += kWave) w.slice[v] = 0;
+
+  const int64_t gw = (int64_t)blockIdx.x * nwaves + wave;
+  const int64_t W = (int64_t)gridDim.x * nwaves;
+  const int64_t n = g.t1 - g.t0;
+  if constexpr (FIRST) {
+    // every tile costs the same: one contiguous run per wave
+    bounded_run<FIRST>(a, g, w, lane, g.t0 + n * gw / W,
+                       g.t0 + n * (gw + 1) / W);
+  } else {
+    // failing tiles come in patches: chunks of kBalanceTiles dealt out
+    // round-robin, so that every wave sees every part of the images
+    for (int64_t c = g.t0 + gw * kBalanceTiles; c < g.t1;
+         c += W * kBalanceTiles)
+      bounded_run<FIRST>(a, g, w, lane, c,
+                         std::min<int64_t>(g.t1, c + kBalanceTiles));
   }
 }
 
@@ -433,15 +450,13 @@ __global__ __launch_bounds__(256) void batch_labels_plain(Group g) {
     int64_t idx[4];
     bool v[4], lvec;
     segment_index<kTileVec>(geo, s - m.tile0, lane, idx, v, lvec);
-    lvec = lvec && (geo.width % 4 == 0) && ((((uintptr_t)geo.pts) & 15u) == 0) &&
-           ((((uintptr_t)m.labels) & 15u) == 0);
+    lvec = lvec && aligned4(geo.pts) && aligned4(m.labels);
     float x[4], y[4], z[4];
     load_colours(geo, idx, v, lvec, x, y, z);
     int lab[4];
     assign_segment<1, false>(cx, cy, cz, cn, k, lane, x, y, z, v, lab, nullptr);
     if (lvec) {
-      *reinterpret_cast<int4*>(m.labels + idx[0]) =
-          make_int4(lab[0], lab[1], lab[2], lab[3]);
+      *reinterpret_cast<I4*>(m.labels + idx[0]) = I4{lab[0], lab[1], lab[2], lab[3]};
     } else {
 #pragma unroll
       for (int j = 0; j < 4; ++j)

@@ -118,6 +118,23 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned x) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// 16-byte vectors that need only 4-byte alignment.  gfx950 global memory
+// takes dword-aligned dwordx4 accesses, so a lane's four consecutive pixels
+// (or labels, or states) move with one instruction whatever the image
+// width or plane offset.
+struct __attribute__((aligned(4))) F4 {
+  float x, y, z, w;
+};
+struct __attribute__((aligned(4))) U4 {
+  uint32_t x, y, z, w;
+};
+struct __attribute__((aligned(4))) I4 {
+  int32_t x, y, z, w;
+};
+__host__ __device__ inline bool aligned4(const void* p) {
+  return ((uintptr_t)p & 3u) == 0;
+}
+
 // -------------------------------------------------- temp-block structs ---
 
 // Lloyd loop state, kept in the caller's temp block.  Written by the last

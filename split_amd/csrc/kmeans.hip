@@ -286,8 +286,8 @@ __global__ __launch_bounds__(1024) void lloyd_step(StepArgs a) {
       assign_segment<CPL, false>(cx, cy, cz, cn, k, lane, x, y, z, v, lab, nullptr);
       if (a.labels) {
         if (lvec) {
-          *reinterpret_cast<int4*>(a.labels + idx[0]) =
-              make_int4(lab[0], lab[1], lab[2], lab[3]);
+          *reinterpret_cast<I4*>(a.labels + idx[0]) =
+              I4{lab[0], lab[1], lab[2], lab[3]};
         } else {
 #pragma unroll
           for (int i = 0; i < 4; ++i)
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(1024) void lloyd_step(StepArgs a) {
       }
     } else {
       if (lvec) {
-        const int4 L = *reinterpret_cast<const int4*>(a.labels_in + idx[0]);
+        const I4 L = *reinterpret_cast<const I4*>(a.labels_in + idx[0]);
         lab[0] = L.x; lab[1] = L.y; lab[2] = L.z; lab[3] = L.w;
       } else {
 #pragma unroll
@@ -413,10 +413,6 @@ static int threads_for(int k) {
   return k <= 128 ? 1024 : k <= 256 ? 512 : k <= 512 ? 256 : 128;
 }
 
-static bool aligned16(const void* p) {
-  return ((uintptr_t)p & 15u) == 0;
-}
-
 struct StepCfg {
   int grid;
   int threads;
@@ -449,11 +445,12 @@ static StepCfg step_cfg(const StepArgs& a, bool accum, size_t partial_cap = 0) {
   } else {
     c.lds = 0;
   }
-  const bool vec = (a.n % 4 == 0) && aligned16(a.pts) &&
-                   (!a.labels || aligned16(a.labels)) &&
-                   (!a.labels_in || aligned16(a.labels_in));
+  // a lane's four consecutive pixels move as one dword-aligned 16-byte
+  // vector, whatever n, the width or the plane offsets
+  const bool vec = aligned4(a.pts) && (!a.labels || aligned4(a.labels)) &&
+                   (!a.labels_in || aligned4(a.labels_in));
   if (a.width > 0)
-    c.mode = (vec && a.width % 4 == 0) ? kTileVec : kTile;
+    c.mode = vec ? kTileVec : kTile;
   else
     c.mode = vec ? kRunVec : kRun;
   return c;

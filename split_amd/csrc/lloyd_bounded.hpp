@@ -88,8 +88,7 @@ __device__ __forceinline__ void commit_assignment(
     ns[i] = pack_state(lab[i],
                        add_down(v[i] ? slack[i] : 0.f, fl[v[i] ? lab[i] : 0]));
   if (lvec) {
-    *reinterpret_cast<uint4*>(a.pstate + idx[0]) =
-        make_uint4(ns[0], ns[1], ns[2], ns[3]);
+    *reinterpret_cast<U4*>(a.pstate + idx[0]) = U4{ns[0], ns[1], ns[2], ns[3]};
   } else {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -257,7 +256,7 @@ __device__ __forceinline__ void walk_tiles(
     auto load_state = [&](int64_t s, bool have, uint32_t (&ps)[4]) {
       int64_t o;
       if (MODE == kTileVec && have && tile_inner(s, o)) {
-        const uint4 q = *reinterpret_cast<const uint4*>(a.pstate + o);
+        const U4 q = *reinterpret_cast<const U4*>(a.pstate + o);
         ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
         return;
       }
@@ -265,7 +264,7 @@ __device__ __forceinline__ void walk_tiles(
       bool v[4], lvec;
       segment_index<MODE>(a, have ? s : 0, lane, idx, v, lvec);
       if (have && lvec) {
-        const uint4 q = *reinterpret_cast<const uint4*>(a.pstate + idx[0]);
+        const U4 q = *reinterpret_cast<const U4*>(a.pstate + idx[0]);
         ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
       } else {
 #pragma unroll

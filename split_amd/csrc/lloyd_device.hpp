@@ -28,9 +28,9 @@ enum Mode { kRun = 0, kRunVec = 1, kTile = 2, kTileVec = 3 };
 constexpr int kTileSide = 16;
 
 // Pixel indices of segment s for this lane: idx[i] of pixel i, v[i] whether
-// it exists, lvec whether the lane's four pixels are consecutive and
-// 16-byte aligned (so per-pixel arrays can be read or written as one
-// 16-byte vector).
+// it exists, lvec whether the lane's four pixels are consecutive and exist
+// (so per-pixel arrays can be read or written as one dword-aligned 16-byte
+// vector).
 template <int MODE>
 __device__ __forceinline__ void segment_index(const Geometry& a, int64_t s,
                                               int lane, int64_t (&idx)[4],
@@ -69,9 +69,9 @@ __device__ __forceinline__ void load_colours(const Geometry& a,
   const float* __restrict__ p2 = a.pts + 2 * a.n;
   if (lvec) {
     const int64_t o = idx[0];
-    const float4 X = *reinterpret_cast<const float4*>(p0 + o);
-    const float4 Y = *reinterpret_cast<const float4*>(p1 + o);
-    const float4 Z = *reinterpret_cast<const float4*>(p2 + o);
+    const F4 X = *reinterpret_cast<const F4*>(p0 + o);
+    const F4 Y = *reinterpret_cast<const F4*>(p1 + o);
+    const F4 Z = *reinterpret_cast<const F4*>(p2 + o);
     x[0] = X.x; x[1] = X.y; x[2] = X.z; x[3] = X.w;
     y[0] = Y.x; y[1] = Y.y; y[2] = Y.z; y[3] = Y.w;
     z[0] = Z.x; z[1] = Z.y; z[2] = Z.z; z[3] = Z.w;

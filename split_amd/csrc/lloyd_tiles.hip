@@ -186,8 +186,8 @@ struct BoundedWalker {
   __device__ __forceinline__ void load_state(const TileGeo& g, bool have,
                                              uint32_t (&ps)[4]) const {
     if (MODE == kTileVec && have && inner(g)) {
-      const uint4 q =
-          *reinterpret_cast<const uint4*>(a.pstate + g.off + lane_off);
+      const U4 q =
+          *reinterpret_cast<const U4*>(a.pstate + g.off + lane_off);
       ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
       return;
     }
@@ -195,7 +195,7 @@ struct BoundedWalker {
     const bool rowok = have && g.row0 + (lane >> 2) < a.height;
     const uint32_t* p = a.pstate + (have ? g.off + lane_off : 0);
     if (MODE == kTileVec && rowok && col + 3 < a.width) {
-      const uint4 q = *reinterpret_cast<const uint4*>(p);
+      const U4 q = *reinterpret_cast<const U4*>(p);
       ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
     } else {
 #pragma unroll
@@ -736,8 +736,8 @@ int cluster_tiles_async(const float* pts, int width, int height,
   const int grid = (int)std::max<int64_t>(
       1, std::min<int64_t>({want, (int64_t)device_cu_count() * per_cu,
                             (int64_t)tiles_max_blocks(k)}));
-  const bool vecmode = vec && (width % 4 == 0) &&
-                       (((uintptr_t)L.pstate & 15u) == 0);
+  // a lane's four pixels move as one dword-aligned 16-byte vector (any width)
+  const bool vecmode = aligned4(pts) && aligned4(L.pstate);
   auto launch = [&](bool full) {
     if (full) {
       if (vecmode) tiles_launch_mode<true, kTileVec>(a, grid, threads, lds, stream);
