@@ -7,8 +7,8 @@ workload: shaded_4096 (4096x4096 Voronoi x Lambertian shading, k=32, stop at
 centroid shift < 1e-4 or 100 iterations, 8-connectivity).
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the bounded
-Lloyd iteration: a tiles_step launch and the one-block tiles_reduce after
-it): 12 bytes/pixel algorithmic traffic (three float planes) per iteration,
+Lloyd iteration, one tiles_step launch that also finalises the previous
+step): 12 bytes/pixel algorithmic traffic (three float planes) per iteration,
 over its average time measured with HIP events on the stream it runs on, as
 the slope of cluster() time between two fixed iteration counts (8 and 40).
 `traffic` is the HBM bytes per bounded tiles_step launch of the same
@@ -121,12 +121,13 @@ def measure_traffic(workload):
             copy = [v for k, v in rows if "copyBuffer" in k]
             steps = [v for k, v in rows if k.startswith("void splithip::tiles_step")
                      and "false>" in k]
-            want = ROOFLINE_HI - 1
+            # launches 2 .. HI run bounded steps; launch HI + 1 only finalises
+            want = ROOFLINE_HI
             if not copy or len(steps) != want:
-                raise RuntimeError(f"{counter}: {len(copy)} copies, {len(steps)} bounded steps "
+                raise RuntimeError(f"{counter}: {len(copy)} copies, {len(steps)} bounded launches "
                                    f"(want {want})")
             scale = COPY_BYTES / (max(copy) * 1024.0)
-            sel = steps[ROOFLINE_LO - 1:]          # iterations LO+1 .. HI
+            sel = steps[ROOFLINE_LO - 1:ROOFLINE_HI - 1]   # iterations LO+1 .. HI
             res[counter] = (sum(sel) / len(sel) * 1024.0 * scale, scale)
     (rd, rs), (wr, ws) = res["FETCH_SIZE"], res["WRITE_SIZE"]
     return {"read": rd, "write": wr, "read_scale": rs, "write_scale": ws}
@@ -666,8 +667,8 @@ def main():
         bytes_per_launch = 12 * n
         achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
         out["roofline"] = {
-            "kernel": "Lloyd iteration: tiles_step<1, 3, false> (skip test, re-assign, exact sums) "
-                      "+ tiles_reduce (totals, centroids, shift, stop flag)",
+            "kernel": "Lloyd iteration: tiles_step<1, 3, false> (finalise the previous "
+                      "step, skip test, re-assign, exact sums)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

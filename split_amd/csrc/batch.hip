@@ -350,8 +350,12 @@ __global__ __launch_bounds__(1024) void batch_step_plain(BatchArgs a, Group g) {
   }
 }
 
-// Tiles per chunk of the bounded iterations' round-robin deal.
-constexpr int64_t kBalanceTiles = 64;
+// Tiles per chunk of the bounded iterations' round-robin deal: smaller
+// chunks balance the waves better, each chunk end costs a partial queue
+// drain and an image flush (batch_256: 224 / 218 / 202 / 207 / 233 ms and
+// the warm-start set: 121 / 109 / 107-111 / 108 / 111 ms at 64 / 96 / 128 /
+// 192 / 256 tiles).
+constexpr int64_t kBalanceTiles = 128;
 
 // The tiles [s, s1) of a group (possibly spanning images) for one wave.
 template <bool FIRST>

@@ -138,7 +138,8 @@ __host__ __device__ inline bool aligned4(const void* p) {
 // -------------------------------------------------- temp-block structs ---
 
 // Lloyd loop state, kept in the caller's temp block.  Written by the last
-// block of each step launch; read by every block of the next launch.
+// block of each step launch (1-D runs path) or by block 0 of each launch's
+// finalisation (tiled path); read by the blocks of later launches.
 struct LloydState {
   unsigned arrive;  // blocks of the current launch that have finished
   int cur;          // which of the two centroid buffers is current
@@ -150,7 +151,7 @@ struct LloydState {
   unsigned next;    // tile chunks handed out in the current launch
 };
 
-// Host-mapped mailbox the last block of a step launch writes, so the host
+// Host-mapped mailbox the finalising block of a launch writes, so the host
 // can stop enqueuing iterations without a device->host copy.
 struct alignas(16) Mailbox {
   int iter;

@@ -34,27 +34,62 @@
 
 namespace splithip {
 
-// totals (tot, [4][k]) -> centroids, shift, bound consumption, stop
-// decision; one block of at least k threads, thread j < k holding the old
-// centroid j (oc) and F[j] (f).
-__device__ void tiles_finalize(const TileArgs& a, const long long* tot,
-                               const float (&oc)[3], float f, float* cent_out,
-                               float fscale, double* d2) {
-  const int k = a.k;
+// Buffers across launches.  Launch s (1-based) assigns with centroids c_s
+// and bound consumption F_s, and adds its integer sums into accumulator
+// A[s % 3] (kAccCopies copies, block b -> copy b % kAccCopies).  Launch
+// s >= 2 first finalises step s - 1 in every block: totals_{s-1} =
+// totals_{s-2} (T[s % 2]) + A[(s - 1) % 3], c_s from the totals, the shift
+// |c_s - c_{s-1}|, F_s and the stop decision; block 0 also publishes them
+// (T[(s - 1) % 2], cbuf[s % 2], fcum[s % 2], state, mailbox) and clears
+// A[(s + 1) % 3] for launch s + 1.  No block of a launch writes what
+// another block of the same launch reads, so one launch per iteration
+// needs no fence or second kernel.
+constexpr int kAccCopies = 8;
+
+__device__ __forceinline__ long long* totals_buf(const TileArgs& a, int j) {
+  return a.totals + (size_t)(j & 1) * 4 * a.k;
+}
+__device__ __forceinline__ long long* acc_buf(const TileArgs& a, int j) {
+  return a.partials + (size_t)(j % 3) * kAccCopies * 4 * a.k;
+}
+
+// Finalisation of step s - 1 at the start of launch s >= 2 (see above).
+// LDS scratch: tot (4k int64), d2 (k doubles), cs (3k floats: c_s on
+// return); fs (k floats) receives F_s.  Returns the stop decision (the same
+// in every block).
+__device__ bool tiles_prologue(const TileArgs& a, int s, float fscale,
+                               long long* tot, double* d2, float* cs,
+                               float* fs) {
+  const int k = a.k, V = 4 * k;
   const int tid = threadIdx.x;
+  {
+    const long long* tp = totals_buf(a, s - 2);
+    const long long* acc = acc_buf(a, s - 1);
+    for (int v = tid; v < V; v += blockDim.x) {
+      long long x[kAccCopies];
+#pragma unroll
+      for (int c = 0; c < kAccCopies; ++c) x[c] = acc[(size_t)c * V + v];
+      long long t = tp[v];
+#pragma unroll
+      for (int c = 0; c < kAccCopies; ++c) t += x[c];
+      tot[v] = t;
+    }
+  }
+  __syncthreads();
+  const float* cold = a.cbuf + (size_t)((s - 1) & 1) * 3 * k;
   const double inv = 1.0 / (double)fscale;
   if (tid < k) {
     const long long cnt = tot[3 * k + tid];
-    double s = 0.0;
+    double sq = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long sum = tot[c * k + tid];
       const float nc = cnt > 0 ? (float)((double)sum * inv / (double)cnt) : 0.f;
-      const double d = (double)nc - (double)oc[c];
-      s += d * d;
-      cent_out[c * k + tid] = nc;
+      const double d = (double)nc - (double)cold[c * k + tid];
+      sq += d * d;
+      cs[c * k + tid] = nc;
     }
-    d2[tid] = s;
+    d2[tid] = sq;
   }
   __syncthreads();
   __shared__ double top[2];
@@ -67,10 +102,10 @@ __device__ void tiles_finalize(const TileArgs& a, const long long* tot,
     for (int j = tid; j < k; j += kWave) part += d2[j];
     const uint64_t pb = (uint64_t)__double_as_longlong(part);
     const uint32_t plo = (uint32_t)pb, phi = (uint32_t)(pb >> 32);
-    double s = 0.0;
+    double sum = 0.0;
 #pragma unroll
     for (int l = 0; l < kWave; ++l)
-      s += __longlong_as_double(
+      sum += __longlong_as_double(
           (long long)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)phi, l)
                        << 32) |
                       (uint32_t)__builtin_amdgcn_readlane((int)plo, l)));
@@ -92,28 +127,39 @@ __device__ void tiles_finalize(const TileArgs& a, const long long* tot,
       if (j != ga) m2 = fmax(m2, d2[j]);
     for (int o = kWave / 2; o > 0; o >>= 1) m2 = fmax(m2, __shfl_xor(m2, o));
     if (tid == 0) {
-      shift_sh = (float)sqrt(s);
+      shift_sh = (float)sqrt(sum);
       top[0] = g1;
       top[1] = m2;
       top_arg = ga;
     }
   }
   __syncthreads();
+  const float* fold = a.fcum + (size_t)((s - 1) & 1) * k;
   if (tid < k)
-    a.fcum[tid] = bound_step(f, d2[tid], tid, top[0], top[1], top_arg);
-  if (tid == 0) {
-    LloydState* st = a.state;
-    const float shift = shift_sh;
-    const int it = st->iter + 1;
-    const int done = !(shift > st->threshold) || it >= st->max_iter;
-    st->iter = it;
-    st->shift = shift;
-    st->cur = 1 - st->cur;
-    st->done = done;
-    if (a.mbox)  // one 16-byte store; the host reads it after a sync
-      *reinterpret_cast<int4*>(a.mbox) =
-          make_int4(it, done, __float_as_int(shift), 0);
+    fs[tid] = bound_step(fold[tid], d2[tid], tid, top[0], top[1], top_arg);
+  LloydState* st = a.state;
+  const float shift = shift_sh;
+  const int it = s - 1;
+  const bool done = !(shift > st->threshold) || it >= st->max_iter;
+  if (blockIdx.x == 0) {
+    long long* tn = totals_buf(a, s - 1);
+    for (int v = tid; v < V; v += blockDim.x) tn[v] = tot[v];
+    long long* an = acc_buf(a, s + 1);
+    for (int v = tid; v < kAccCopies * V; v += blockDim.x) an[v] = 0;
+    float* cn = a.cbuf + (size_t)(s & 1) * 3 * k;
+    for (int v = tid; v < 3 * k; v += blockDim.x) cn[v] = cs[v];
+    if (tid < k) a.fcum[(size_t)(s & 1) * k + tid] = fs[tid];
+    if (tid == 0) {
+      st->iter = it;
+      st->shift = shift;
+      st->cur = s & 1;
+      st->done = done;
+      if (a.mbox)  // one 16-byte store; the host reads it after a sync
+        *reinterpret_cast<int4*>(a.mbox) =
+            make_int4(it, done, __float_as_int(shift), 0);
+    }
   }
+  return done;
 }
 
 // ------------------------------------------------ bounded iterations ---
@@ -375,28 +421,49 @@ struct BoundedWalker {
   }
 };
 
-// Copies of the step accumulator (blocks add into copy blockIdx % 8, so
-// that fewer blocks contend for one address).
-constexpr int kAccCopies = 8;
 
+// Launch s of a tiled run (FULL: s == 1, every pixel assigned).
 template <int CPL, int MODE, bool FULL>
-__global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
+__global__ __launch_bounds__(1024) void tiles_step(TileArgs a, int s) {
   extern __shared__ long long lds64[];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int nwaves = blockDim.x / kWave;
   const int k = a.k;
-  const LloydState* st = a.state;
-  if (st->done) return;
-  const float* cent = a.cbuf + (size_t)st->cur * 3 * k;
+  // done only ever goes 0 -> 1, and a block that sees it set by block 0 of
+  // this launch would have reached the same decision; read once per block so
+  // that the whole block returns (or not) together
+  __shared__ int done_sh;
+  if (threadIdx.x == 0) done_sh = a.state->done;
+  __syncthreads();
+  if (done_sh) return;
 
   const int V = 4 * k;
   long long* slices = lds64;                                   // nwaves x V
   float* fl = reinterpret_cast<float*>(lds64 + nwaves * V);    // F[0..k)
   int* qidx = reinterpret_cast<int*>(fl + kMaxTileK) + wave * 2 * kQueueCap;
   int* qold = qidx + kQueueCap;
+  int2* list = reinterpret_cast<int2*>(
+      reinterpret_cast<int*>(fl + kMaxTileK) + nwaves * 2 * kQueueCap);
+  const float maxabs = __uint_as_float(*a.maxabs);
+  const float fscale = scale_of(*a.maxabs);
+  const float margin = 1e-5f * (1.f + 4.f * maxabs);
+
+  // this launch's centroids c_s (and F_s in fl): the initial ones for s = 1,
+  // else finalised here from step s - 1 (scratch in the unit-list area,
+  // 8 KiB + 2 KiB + 3 KiB at k = 256 < 16 KiB)
+  const float* cent;
+  if (s == 1) {
+    cent = a.cbuf + 3 * k;  // cbuf[1] holds c_1, fcum[1] F_1 = 0
+    for (int i = threadIdx.x; i < k; i += blockDim.x) fl[i] = a.fcum[k + i];
+  } else {
+    long long* tot = reinterpret_cast<long long*>(list);
+    double* d2 = reinterpret_cast<double*>(tot + 4 * kMaxTileK);
+    float* cs = reinterpret_cast<float*>(d2 + kMaxTileK);
+    if (tiles_prologue(a, s, fscale, tot, d2, cs, fl)) return;
+    cent = cs;
+  }
   for (int i = threadIdx.x; i < nwaves * V; i += blockDim.x) slices[i] = 0;
-  for (int i = threadIdx.x; i < k; i += blockDim.x) fl[i] = a.fcum[i];
 
   float cx[CPL], cy[CPL], cz[CPL], cn[CPL];
 #pragma unroll
@@ -408,10 +475,7 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
     cz[t] = live ? cent[2 * k + j] : 0.f;
     cn[t] = fmaf(cz[t], cz[t], fmaf(cy[t], cy[t], cx[t] * cx[t]));
   }
-  const float maxabs = __uint_as_float(*a.maxabs);
-  const float fscale = scale_of(*a.maxabs);
-  const float margin = 1e-5f * (1.f + 4.f * maxabs);
-  __syncthreads();
+  __syncthreads();  // cent (LDS for s >= 2) is read; the list area is free
   long long* slice = slices + (size_t)wave * V;
 
   if constexpr (FULL) {
@@ -423,8 +487,6 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
         a, TileRange{a.nseg * gw / W, a.nseg * (gw + 1) / W}, cx, cy, cz, cn,
         fl, fscale, margin, slice, qidx, qold, lane);
   } else {
-    int2* list = reinterpret_cast<int2*>(
-        reinterpret_cast<int*>(fl + kMaxTileK) + nwaves * 2 * kQueueCap);
     __shared__ unsigned unit_next;
     BoundedWalker<CPL, MODE> w{a,     cx,    cy,   cz,   cn,
                                fl,    fscale, margin, slice, qidx,
@@ -444,10 +506,10 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
     w.finish_block(reinterpret_cast<int*>(fl + kMaxTileK), wave, nwaves);
   }
   // the block's share of the step, summed over its waves, is added (device
-  // atomics) into copy blockIdx % kAccCopies of the step accumulator;
-  // tiles_reduce adds the copies into the totals after this launch ends
+  // atomics) into copy blockIdx % kAccCopies of accumulator A[s % 3]; the
+  // next launch's prologue adds the copies into the totals
   __syncthreads();
-  long long* acc = a.partials + (size_t)(blockIdx.x % kAccCopies) * V;
+  long long* acc = acc_buf(a, s) + (size_t)(blockIdx.x % kAccCopies) * V;
   for (int vv = threadIdx.x; vv < V; vv += blockDim.x) {
     long long tot = 0;
     for (int w = 0; w < nwaves; ++w) tot += slices[(size_t)w * V + vv];
@@ -457,55 +519,18 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a) {
   }
 }
 
-// One block after each step launch: totals += the accumulator copies
-// (integer sums), which it clears for the next step, then tiles_finalize.
-constexpr int kReduceThreads = 1024;
-
-__global__ __launch_bounds__(kReduceThreads) void tiles_reduce(TileArgs a) {
-  __shared__ long long tot[4 * kMaxTileK];
-  __shared__ double d2[kMaxTileK];
-  LloydState* st = a.state;
-  if (st->done) return;
-  const int k = a.k, V = 4 * k;
-  const int tid = threadIdx.x;
-  static_assert(4 * kMaxTileK <= kReduceThreads, "one entry per thread");
-  static_assert(kMaxTileK <= kReduceThreads, "one centroid per thread");
-  const int cur = st->cur;
-  const float* cent = a.cbuf + (size_t)cur * 3 * k;
-  float* cent_out = a.cbuf + (size_t)(1 - cur) * 3 * k;
-  float oc[3] = {0.f, 0.f, 0.f}, f = 0.f;
-  if (tid < k) {
-    oc[0] = cent[tid];
-    oc[1] = cent[k + tid];
-    oc[2] = cent[2 * k + tid];
-    f = a.fcum[tid];
-  }
-  if (tid < V) {
-    long long t = a.totals[tid];
-    long long x[kAccCopies];
-#pragma unroll
-    for (int c = 0; c < kAccCopies; ++c) x[c] = a.partials[(size_t)c * V + tid];
-#pragma unroll
-    for (int c = 0; c < kAccCopies; ++c) {
-      t += x[c];
-      a.partials[(size_t)c * V + tid] = 0;  // for the next step
-    }
-    a.totals[tid] = t;
-    tot[tid] = t;
-  }
-  __syncthreads();
-  tiles_finalize(a, tot, oc, f, cent_out, scale_of(*a.maxabs), d2);
-}
-
+// c_1 (the initial centroids) into cbuf[1], zero totals, accumulators and
+// F (both buffers).
 __global__ void tiles_init(LloydState* st, float* cbuf, const float* cent,
                            int k, int max_iter, float threshold,
                            long long* totals, float* fcum, unsigned* maxabs) {
-  for (int i = threadIdx.x; i < 3 * k; i += blockDim.x) cbuf[i] = cent[i];
-  for (int i = threadIdx.x; i < 4 * k * (1 + kAccCopies); i += blockDim.x)
-    totals[i] = 0;  // totals, then the step accumulator copies
-  for (int i = threadIdx.x; i < k; i += blockDim.x) fcum[i] = 0.f;
+  for (int i = threadIdx.x; i < 3 * k; i += blockDim.x)
+    cbuf[3 * k + i] = cent[i];
+  for (int i = threadIdx.x; i < 4 * k * (2 + 3 * kAccCopies); i += blockDim.x)
+    totals[i] = 0;  // two totals buffers, then the three accumulators
+  for (int i = threadIdx.x; i < 2 * k; i += blockDim.x) fcum[i] = 0.f;
   if (threadIdx.x == 0) {
-    st->cur = 0;
+    st->cur = 1;
     st->iter = 0;
     st->done = max_iter <= 0;
     st->max_iter = max_iter;
@@ -607,11 +632,12 @@ static TileLayout tiles_layout(void* temp, int64_t n, int k) {
   };
   L.st = reinterpret_cast<LloydState*>(take(sizeof(LloydState)));
   L.cbuf = reinterpret_cast<float*>(take(6 * (size_t)k * sizeof(float)));
-  // totals and the kAccCopies accumulator copies, contiguous
+  // two totals buffers and three accumulators of kAccCopies copies each,
+  // contiguous
   L.totals = reinterpret_cast<long long*>(
-      take((size_t)(1 + kAccCopies) * 4 * k * 8));
-  L.partials = L.totals ? L.totals + 4 * (size_t)k : nullptr;
-  L.fcum = reinterpret_cast<float*>(take((size_t)k * sizeof(float)));
+      take((size_t)(2 + 3 * kAccCopies) * 4 * k * 8));
+  L.partials = L.totals ? L.totals + 2 * 4 * (size_t)k : nullptr;
+  L.fcum = reinterpret_cast<float*>(take(2 * (size_t)k * sizeof(float)));
   L.maxabs = reinterpret_cast<unsigned*>(take(sizeof(unsigned)));
   L.pstate = reinterpret_cast<uint32_t*>(take((size_t)n * sizeof(uint32_t)));
   L.bytes = o;
@@ -654,20 +680,20 @@ bool tiles_supported(int64_t n, int width, int k) {
 }
 
 template <bool FULL, int MODE>
-static void tiles_launch_mode(const TileArgs& a, int grid, int threads,
-                              size_t lds, hipStream_t stream) {
+static void tiles_launch_mode(const TileArgs& a, int s, int grid,
+                              int threads, size_t lds, hipStream_t stream) {
   switch (a.k <= 64 ? 1 : a.k <= 128 ? 2 : 4) {
     case 1:
       hipLaunchKernelGGL((tiles_step<1, MODE, FULL>), dim3(grid),
-                         dim3(threads), lds, stream, a);
+                         dim3(threads), lds, stream, a, s);
       break;
     case 2:
       hipLaunchKernelGGL((tiles_step<2, MODE, FULL>), dim3(grid),
-                         dim3(threads), lds, stream, a);
+                         dim3(threads), lds, stream, a, s);
       break;
     default:
       hipLaunchKernelGGL((tiles_step<4, MODE, FULL>), dim3(grid),
-                         dim3(threads), lds, stream, a);
+                         dim3(threads), lds, stream, a, s);
       break;
   }
 }
@@ -738,23 +764,25 @@ int cluster_tiles_async(const float* pts, int width, int height,
                             (int64_t)tiles_max_blocks(k)}));
   // a lane's four pixels move as one dword-aligned 16-byte vector (any width)
   const bool vecmode = aligned4(pts) && aligned4(L.pstate);
-  auto launch = [&](bool full) {
-    if (full) {
-      if (vecmode) tiles_launch_mode<true, kTileVec>(a, grid, threads, lds, stream);
-      else tiles_launch_mode<true, kTile>(a, grid, threads, lds, stream);
-    } else {
-      if (vecmode) tiles_launch_mode<false, kTileVec>(a, grid, threads, lds, stream);
-      else tiles_launch_mode<false, kTile>(a, grid, threads, lds, stream);
-    }
-    hipLaunchKernelGGL(tiles_reduce, dim3(1), dim3(kReduceThreads), 0, stream,
-                       a);
+  // launch s = 1 .. max_iter + 1: launch s finalises step s - 1 and runs
+  // step s; the last one only finalises (it stops at the iteration limit)
+  auto launch = [&](int s) {
+    if (s == 1 && vecmode)
+      tiles_launch_mode<true, kTileVec>(a, s, grid, threads, lds, stream);
+    else if (s == 1)
+      tiles_launch_mode<true, kTile>(a, s, grid, threads, lds, stream);
+    else if (vecmode)
+      tiles_launch_mode<false, kTileVec>(a, s, grid, threads, lds, stream);
+    else
+      tiles_launch_mode<false, kTile>(a, s, grid, threads, lds, stream);
   };
 
   constexpr int kChunk = 4;
+  const int total = max_iter > 0 ? max_iter + 1 : 0;
   int launched = 0, chunk = 0;
-  while (launched < max_iter) {
-    const int cnt = std::min(kChunk, max_iter - launched);
-    for (int i = 0; i < cnt; ++i) launch(launched + i == 0);
+  while (launched < total) {
+    const int cnt = std::min(kChunk, total - launched);
+    for (int i = 0; i < cnt; ++i) launch(launched + i + 1);
     SPLIT_HIP_CHECK_LAUNCH("tiles_step");
     launched += cnt;
     SPLIT_HIP_CHECK(hipEventRecord(res->ev[chunk & 1], stream));
