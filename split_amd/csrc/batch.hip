@@ -119,7 +119,7 @@ __device__ __forceinline__ int find_image(const BatchImg* img, int count,
 }
 
 __device__ __forceinline__ Geometry image_geometry(const BatchImg& m) {
-  Geometry g;
+  Geometry g{};
   g.pts = m.pts;
   g.n = (int64_t)m.W * m.H;
   g.width = m.W;

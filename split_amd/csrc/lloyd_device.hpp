@@ -18,6 +18,10 @@ struct Geometry {
   int width;         // > 0: 16x16-tile traversal of width x height
   int height;
   int tiles_x;       // ceil(width / 16)
+  // optional interleaved copy of the pixels ({x, y, z, 0} per pixel) for
+  // scattered reads: one 16-byte load touches one cache line instead of
+  // three (one per plane)
+  const F4* aos = nullptr;
 };
 
 // ------------------------------------------------------------- loading ---
@@ -75,6 +79,14 @@ __device__ __forceinline__ void load_colours(const Geometry& a,
     x[0] = X.x; x[1] = X.y; x[2] = X.z; x[3] = X.w;
     y[0] = Y.x; y[1] = Y.y; y[2] = Y.z; y[3] = Y.w;
     z[0] = Z.x; z[1] = Z.y; z[2] = Z.z; z[3] = Z.w;
+  } else if (a.aos) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const F4 q = a.aos[v[i] ? idx[i] : 0];
+      x[i] = q.x;
+      y[i] = q.y;
+      z[i] = q.z;
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {

@@ -580,6 +580,44 @@ __global__ __launch_bounds__(256) void tiles_absmax(const float* pts,
   }
 }
 
+// tiles_absmax plus the interleaved copy of the pixels ({x, y, z, 0}) that
+// the queued pixels are read from.
+__global__ __launch_bounds__(256) void tiles_absmax_pack(const float* pts,
+                                                         int64_t n, F4* aos,
+                                                         unsigned* out) {
+  float m = 0.f;
+  const int64_t quads = n / 4;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < quads;
+       q += stride) {
+    const int64_t i = 4 * q;
+    const F4 X = *reinterpret_cast<const F4*>(pts + i);
+    const F4 Y = *reinterpret_cast<const F4*>(pts + n + i);
+    const F4 Z = *reinterpret_cast<const F4*>(pts + 2 * n + i);
+    aos[i] = F4{X.x, Y.x, Z.x, 0.f};
+    aos[i + 1] = F4{X.y, Y.y, Z.y, 0.f};
+    aos[i + 2] = F4{X.z, Y.z, Z.z, 0.f};
+    aos[i + 3] = F4{X.w, Y.w, Z.w, 0.f};
+    m = fmaxf(m, fmaxf(fmaxf(fmaxf(fabsf(X.x), fabsf(X.y)), fmaxf(fabsf(X.z), fabsf(X.w))),
+                       fmaxf(fmaxf(fmaxf(fabsf(Y.x), fabsf(Y.y)), fmaxf(fabsf(Y.z), fabsf(Y.w))),
+                             fmaxf(fmaxf(fabsf(Z.x), fabsf(Z.y)), fmaxf(fabsf(Z.z), fabsf(Z.w))))));
+  }
+  for (int64_t i = 4 * quads + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       i < n; i += stride) {
+    const float x = pts[i], y = pts[n + i], z = pts[2 * n + i];
+    aos[i] = F4{x, y, z, 0.f};
+    m = fmaxf(m, fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z))));
+  }
+  __shared__ float part[256 / kWave];
+  m = wave_allreduce(m, OpMax());
+  if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)blockDim.x / kWave; ++w) m = fmaxf(m, part[w]);
+    atomicMax(out, __float_as_uint(m));
+  }
+}
+
 __global__ void tiles_labels(const LloydState* st, const uint32_t* pstate,
                              int32_t* labels, int64_t n) {
   if (st->iter == 0) return;
@@ -618,6 +656,7 @@ struct TileLayout {
   float* fcum;
   unsigned* maxabs;
   uint32_t* pstate;
+  F4* aos;
   size_t bytes;
 };
 
@@ -640,6 +679,7 @@ static TileLayout tiles_layout(void* temp, int64_t n, int k) {
   L.fcum = reinterpret_cast<float*>(take(2 * (size_t)k * sizeof(float)));
   L.maxabs = reinterpret_cast<unsigned*>(take(sizeof(unsigned)));
   L.pstate = reinterpret_cast<uint32_t*>(take((size_t)n * sizeof(uint32_t)));
+  L.aos = reinterpret_cast<F4*>(take((size_t)n * sizeof(F4)));
   L.bytes = o;
   return L;
 }
@@ -717,10 +757,9 @@ int cluster_tiles_async(const float* pts, int width, int height,
   hipLaunchKernelGGL(tiles_init, dim3(1), dim3(256), 0, stream, L.st, L.cbuf,
                      centroids, k, max_iter, threshold, L.totals, L.fcum,
                      L.maxabs);
-  const int vec = ((uintptr_t)pts & 15u) == 0;
-  hipLaunchKernelGGL(tiles_absmax, dim3(8 * device_cu_count()), dim3(256), 0,
-                     stream, pts, 3 * n, vec, L.maxabs);
-  SPLIT_HIP_CHECK_LAUNCH("tiles_init/tiles_absmax");
+  hipLaunchKernelGGL(tiles_absmax_pack, dim3(8 * device_cu_count()), dim3(256),
+                     0, stream, pts, n, L.aos, L.maxabs);
+  SPLIT_HIP_CHECK_LAUNCH("tiles_init/tiles_absmax_pack");
 
   TileArgs a{};
   a.pts = pts;
@@ -735,6 +774,7 @@ int cluster_tiles_async(const float* pts, int width, int height,
   a.partials = L.partials;
   a.fcum = L.fcum;
   a.pstate = L.pstate;
+  a.aos = L.aos;
   a.maxabs = L.maxabs;
   a.state = L.st;
   a.mbox = res->dev;
