@@ -225,8 +225,10 @@ __device__ __forceinline__ void assign_segment(
       for (int i = 0; i < 4; ++i) {
         const float d = sqdist(x[i], y[i], z[i], ccx, ccy, ccz);
         const bool better = d < bd[i];
-        if constexpr (TOP2) bd2[i] = better ? bd[i] : fminf(bd2[i], d);
-        bd[i] = better ? d : bd[i];
+        // with bd <= bd2 always, the median of (bd, d, bd2) is the new
+        // runner-up: one v_med3_f32
+        if constexpr (TOP2) bd2[i] = __builtin_amdgcn_fmed3f(bd[i], d, bd2[i]);
+        bd[i] = fminf(bd[i], d);
         bk[i] = better ? j : bk[i];
       }
     }
