@@ -174,6 +174,9 @@ __device__ bool tiles_prologue(const TileArgs& a, int s, float fscale,
 // through an LDS counter.  Pixel offsets are int32 (the tiled path
 // requires n < 2^31).
 constexpr int kBlockUnits = 2048;
+// Entries after the last pixel's state that the walker's 16-byte lane loads
+// may read (and ignore).
+constexpr int kStatePad = 16;
 constexpr int kMaxQueueWaves = 16;  // waves per block (1024 threads)
 
 // One tile: offset of its first pixel, first row and column.
@@ -229,25 +232,17 @@ struct BoundedWalker {
     return g.row0 + kTileSide <= a.height && g.col0 + kTileSide <= a.width;
   }
 
+  // One 16-byte load per lane on every path, so that the wait for this
+  // tile's state (issued a tile earlier) never has to cover the next tile's
+  // load as well.  Lanes below the image read the tile's first row, lanes
+  // past its right edge read on into the next row or the padding after the
+  // last pixel (at most 15 entries, kStatePad); visit() masks such pixels.
   __device__ __forceinline__ void load_state(const TileGeo& g, bool have,
                                              uint32_t (&ps)[4]) const {
-    if (MODE == kTileVec && have && inner(g)) {
-      const U4 q =
-          *reinterpret_cast<const U4*>(a.pstate + g.off + lane_off);
-      ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
-      return;
-    }
-    const int col = g.col0 + 4 * (lane & 3);
-    const bool rowok = have && g.row0 + (lane >> 2) < a.height;
-    const uint32_t* p = a.pstate + (have ? g.off + lane_off : 0);
-    if (MODE == kTileVec && rowok && col + 3 < a.width) {
-      const U4 q = *reinterpret_cast<const U4*>(p);
-      ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        ps[i] = rowok && col + i < a.width ? p[i] : 0u;
-    }
+    const bool rowok = g.row0 + (lane >> 2) < a.height;
+    const int o = have ? g.off + (rowok ? lane_off : 4 * (lane & 3)) : 0;
+    const U4 q = *reinterpret_cast<const U4*>(a.pstate + o);
+    ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
   }
 
   // Re-assigns the first min(qlen, 256) queued pixels, then moves the rest
@@ -678,7 +673,8 @@ static TileLayout tiles_layout(void* temp, int64_t n, int k) {
   L.partials = L.totals ? L.totals + 2 * 4 * (size_t)k : nullptr;
   L.fcum = reinterpret_cast<float*>(take(2 * (size_t)k * sizeof(float)));
   L.maxabs = reinterpret_cast<unsigned*>(take(sizeof(unsigned)));
-  L.pstate = reinterpret_cast<uint32_t*>(take((size_t)n * sizeof(uint32_t)));
+  L.pstate = reinterpret_cast<uint32_t*>(
+      take(((size_t)n + kStatePad) * sizeof(uint32_t)));
   L.aos = reinterpret_cast<F4*>(take((size_t)n * sizeof(F4)));
   L.bytes = o;
   return L;
