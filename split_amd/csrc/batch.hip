@@ -691,7 +691,7 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
     m.totals = reinterpret_cast<long long*>(take(4 * (size_t)e.k * 8));
     m.bitmap = reinterpret_cast<unsigned*>(take(2 * (size_t)((n + 63) / 64) * 4));
     m.prefix = reinterpret_cast<unsigned*>(take((size_t)((n + 31) / 32) * 4));
-    m.pstate = reinterpret_cast<uint32_t*>(take((size_t)n * 4));
+    m.pstate = reinterpret_cast<uint32_t*>(take(((size_t)n + kStatePad) * 4));
     m.fcum = reinterpret_cast<float*>(take((size_t)e.k * sizeof(float)));
   }
   if (P->nplain == count) P->tiles_plain = tiles;

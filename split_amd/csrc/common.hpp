@@ -135,6 +135,17 @@ __host__ __device__ inline bool aligned4(const void* p) {
   return ((uintptr_t)p & 3u) == 0;
 }
 
+// Global-memory view of a generic pointer.  Pointers that reach a kernel
+// through memory (the batched path's per-image records) are generic to the
+// compiler, and its flat accesses count against both the vector-memory and
+// the LDS wait counters, so every LDS wait would also wait for outstanding
+// loads; accesses through gmem() compile to global_* instructions.  Only for
+// pointers into device memory.
+template <class T>
+__device__ __forceinline__ T __attribute__((address_space(1)))* gmem(T* p) {
+  return (T __attribute__((address_space(1)))*)p;
+}
+
 // -------------------------------------------------- temp-block structs ---
 
 // Lloyd loop state, kept in the caller's temp block.  Written by the last

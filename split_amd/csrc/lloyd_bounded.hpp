@@ -88,11 +88,12 @@ __device__ __forceinline__ void commit_assignment(
     ns[i] = pack_state(lab[i],
                        add_down(v[i] ? slack[i] : 0.f, fl[v[i] ? lab[i] : 0]));
   if (lvec) {
-    *reinterpret_cast<U4*>(a.pstate + idx[0]) = U4{ns[0], ns[1], ns[2], ns[3]};
+    *gmem(reinterpret_cast<U4*>(a.pstate + idx[0])) =
+        U4{ns[0], ns[1], ns[2], ns[3]};
   } else {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      if (v[i]) a.pstate[idx[i]] = ns[i];
+      if (v[i]) gmem(a.pstate)[idx[i]] = ns[i];
   }
   int qx[4], qy[4], qz[4];
 #pragma unroll
@@ -188,6 +189,9 @@ struct TileRange {
 };
 
 constexpr int kQueueCap = 512;      // per-wave queue of failing pixels
+// Entries after an image's last pixel state that the walkers' 16-byte lane
+// loads may read (and ignore); state buffers are allocated with them.
+constexpr int kStatePad = 16;
 constexpr int kWholeTileFails = 96;  // more failing pixels: re-assign the tile
 
 // Processes, for one wave, the tiles that next(t) hands out until it
@@ -244,6 +248,7 @@ __device__ __forceinline__ void walk_tiles(
                           old, slice, run);
     }
   } else {
+    static_assert(MODE == kTile || MODE == kTileVec, "bounded walks cover tiles");
     // Tile s wholly inside the image (wave-uniform); o = offset of this
     // lane's first pixel (tile indices of one image fit in int).
     auto tile_inner = [&](int64_t s, int64_t& o) {
@@ -253,23 +258,20 @@ __device__ __forceinline__ void walk_tiles(
           4 * (lane & 3);
       return (ty + 1) * kTileSide <= a.height && (tx + 1) * kTileSide <= a.width;
     };
+    // One 16-byte load per lane on every path, so that the wait for a
+    // tile's state (issued a tile earlier) never covers the next tile's
+    // load: lanes below the image read the tile's first row, lanes past its
+    // right edge read on into the next row or the kStatePad entries after
+    // the last pixel; visit() masks such pixels.
     auto load_state = [&](int64_t s, bool have, uint32_t (&ps)[4]) {
-      int64_t o;
-      if (MODE == kTileVec && have && tile_inner(s, o)) {
-        const U4 q = *reinterpret_cast<const U4*>(a.pstate + o);
-        ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
-        return;
-      }
-      int64_t idx[4];
-      bool v[4], lvec;
-      segment_index<MODE>(a, have ? s : 0, lane, idx, v, lvec);
-      if (have && lvec) {
-        const U4 q = *reinterpret_cast<const U4*>(a.pstate + idx[0]);
-        ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) ps[i] = have && v[i] ? a.pstate[idx[i]] : 0u;
-      }
+      const int t = have ? (int)s : 0;
+      const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
+      const int row = ty * kTileSide + (lane >> 2);
+      const int64_t o =
+          (int64_t)(row < a.height ? row : ty * kTileSide) * a.width +
+          tx * kTileSide + 4 * (lane & 3);
+      const U4 q = *gmem(reinterpret_cast<const U4*>(a.pstate + o));
+      ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
     };
     auto visit = [&](int64_t s, const uint32_t (&ps)[4]) {
       int64_t idx[4];

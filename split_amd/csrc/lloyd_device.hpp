@@ -68,21 +68,21 @@ __device__ __forceinline__ void load_colours(const Geometry& a,
                                              const bool (&v)[4], bool lvec,
                                              float (&x)[4], float (&y)[4],
                                              float (&z)[4]) {
-  const float* __restrict__ p0 = a.pts;
-  const float* __restrict__ p1 = a.pts + a.n;
-  const float* __restrict__ p2 = a.pts + 2 * a.n;
+  const auto p0 = gmem(a.pts);
+  const auto p1 = gmem(a.pts + a.n);
+  const auto p2 = gmem(a.pts + 2 * a.n);
   if (lvec) {
     const int64_t o = idx[0];
-    const F4 X = *reinterpret_cast<const F4*>(p0 + o);
-    const F4 Y = *reinterpret_cast<const F4*>(p1 + o);
-    const F4 Z = *reinterpret_cast<const F4*>(p2 + o);
+    const F4 X = *gmem(reinterpret_cast<const F4*>(a.pts + o));
+    const F4 Y = *gmem(reinterpret_cast<const F4*>(a.pts + a.n + o));
+    const F4 Z = *gmem(reinterpret_cast<const F4*>(a.pts + 2 * a.n + o));
     x[0] = X.x; x[1] = X.y; x[2] = X.z; x[3] = X.w;
     y[0] = Y.x; y[1] = Y.y; y[2] = Y.z; y[3] = Y.w;
     z[0] = Z.x; z[1] = Z.y; z[2] = Z.z; z[3] = Z.w;
   } else if (a.aos) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const F4 q = a.aos[v[i] ? idx[i] : 0];
+      const F4 q = gmem(a.aos)[v[i] ? idx[i] : 0];
       x[i] = q.x;
       y[i] = q.y;
       z[i] = q.z;

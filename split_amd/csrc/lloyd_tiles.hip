@@ -174,9 +174,6 @@ __device__ bool tiles_prologue(const TileArgs& a, int s, float fscale,
 // through an LDS counter.  Pixel offsets are int32 (the tiled path
 // requires n < 2^31).
 constexpr int kBlockUnits = 2048;
-// Entries after the last pixel's state that the walker's 16-byte lane loads
-// may read (and ignore).
-constexpr int kStatePad = 16;
 constexpr int kMaxQueueWaves = 16;  // waves per block (1024 threads)
 
 // One tile: offset of its first pixel, first row and column.
