@@ -140,11 +140,19 @@ __host__ __device__ inline bool aligned4(const void* p) {
 // compiler, and its flat accesses count against both the vector-memory and
 // the LDS wait counters, so every LDS wait would also wait for outstanding
 // loads; accesses through gmem() compile to global_* instructions.  Only for
-// pointers into device memory.
+// pointers into device memory.  (The host pass, which only parses device
+// code, sees the identity.)
+#if defined(__HIP_DEVICE_COMPILE__)
 template <class T>
 __device__ __forceinline__ T __attribute__((address_space(1)))* gmem(T* p) {
   return (T __attribute__((address_space(1)))*)p;
 }
+#else
+template <class T>
+__device__ __forceinline__ T* gmem(T* p) {
+  return p;
+}
+#endif
 
 // -------------------------------------------------- temp-block structs ---
 
