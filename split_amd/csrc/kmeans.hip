@@ -529,9 +529,14 @@ int driver_res(DriverRes** out) {
   }
   if (r.device != dev) {
     SPLIT_HIP_CHECK(hipHostGetDevicePointer((void**)&r.dev, r.host, 0));
+    // The events only pace the host (it waits for chunk c - 1 while chunk c
+    // runs, then reads the mailbox): without the system-scope fence an event
+    // costs no cache writeback and invalidation between two Lloyd launches
+    // (~6 us each).  A mailbox read that lags only enqueues launches that
+    // return at once; results are read after a full stream synchronisation.
     for (int i = 0; i < 2; ++i)
-      SPLIT_HIP_CHECK(
-          hipEventCreateWithFlags(&r.ev[i], hipEventDisableTiming));
+      SPLIT_HIP_CHECK(hipEventCreateWithFlags(
+          &r.ev[i], hipEventDisableTiming | hipEventDisableSystemFence));
     r.device = dev;
   }
   *out = &r;

@@ -192,7 +192,12 @@ constexpr int kQueueCap = 512;      // per-wave queue of failing pixels
 // Entries after an image's last pixel state that the walkers' 16-byte lane
 // loads may read (and ignore); state buffers are allocated with them.
 constexpr int kStatePad = 16;
-constexpr int kWholeTileFails = 96;  // more failing pixels: re-assign the tile
+// More failing pixels: re-assign the tile whole (its compact colour box keeps
+// ~7 candidates) rather than queue them (a queued batch evaluates nearly all
+// k).  On the 4096² k = 32 image and the 256-image batch: 64 / 96 / 128 /
+// 160 / 224 / never -> 71.5 / 70.5 / 71.6 / 69.6 / 70.7 / 70.9 us per
+// iteration and 202 / 194 / 193 / 191 / 191 / 196 ms.
+constexpr int kWholeTileFails = 160;
 
 // Processes, for one wave, the tiles that next(t) hands out until it
 // returns false: FULL re-assigns every pixel (run accumulation), otherwise

@@ -154,7 +154,7 @@ __device__ bool tiles_prologue(const TileArgs& a, int s, float fscale,
       st->shift = shift;
       st->cur = s & 1;
       st->done = done;
-      if (a.mbox)  // one 16-byte store; the host reads it after a sync
+      if (a.mbox)  // one 16-byte store; the host polls it between chunks
         *reinterpret_cast<int4*>(a.mbox) =
             make_int4(it, done, __float_as_int(shift), 0);
     }
