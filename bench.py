@@ -14,6 +14,9 @@ the slope of cluster() time between two fixed iteration counts (8 and 40).
 `traffic` is the HBM bytes per bounded tiles_step launch of the same
 iterations, from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) of
 `bench.py --profile-driver`, scaled by the counters of a 256 MiB copy.
+`roofline.valu_dense_equiv` puts the same iteration against the vector ALU:
+the dense assignment's 6 lane-ops per pixel and centroid over the measured
+time, as a fraction of the unpacked fp32 issue rate.
 `cpu_baseline` times the oracle's restatement of the reference pipeline on a
 bounded crop of the same image, single-threaded, on this host.
 
@@ -39,6 +42,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# unpacked fp32 vector issue: 256 CUs x 64 lanes x 2.4 GHz (lane-ops/s, 1e12)
+VALU_PEAK_TOPS = 39.32
 
 
 def parse():
@@ -677,6 +682,17 @@ def main():
             "traffic": None,
             "per_launch_us": round(per_launch_ms * 1e3, 2),
             "bytes_per_launch": bytes_per_launch,
+            # the VALU side for k >= 32: the dense direct-form assignment's
+            # lane operations (3 sub + 1 mul + 2 fma per pixel and centroid)
+            # per iteration over the same time, against gfx950's unpacked
+            # fp32 issue rate; above 1 = the bounds skip more work than the
+            # dense form could do at peak
+            "valu_dense_equiv": {
+                "lane_ops": 6 * n * w.k,
+                "achieved_tops": round(6 * n * w.k / (per_launch_ms / 1e3) / 1e12, 2),
+                "peak_tops": VALU_PEAK_TOPS,
+                "frac": round(6 * n * w.k / (per_launch_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 3),
+            },
         }
         log(f"bench: lloyd_step {per_launch_ms * 1e3:.2f} us/launch -> {achieved:.1f} GB/s")
         if not args.no_traffic:
