@@ -18,7 +18,8 @@ iterations, from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) of
 the dense assignment's 6 lane-ops per pixel and centroid over the measured
 time, as a fraction of the unpacked fp32 issue rate.
 `cpu_baseline` times the oracle's restatement of the reference pipeline on a
-bounded crop of the same image, single-threaded, on this host.
+bounded crop of the same image on this host: on all OMP_NUM_THREADS threads
+(the oracle's OpenMP build; `cores`), and on one thread (`single_thread`).
 
 `--workload batch_256` instead runs the batched path (256 images of uneven
 size and k, one graph-replayed launch sequence) against the same images
@@ -530,21 +531,33 @@ def cpu_baseline(img, width, height, w, gpu_iters, crop):
     sub = np.ascontiguousarray(planes.reshape(3, c * c))
     idx = oracle.sample_indices(w.init_seed, c * c, w.k)
     init = np.ascontiguousarray(sub[:, idx])
-    t0 = time.perf_counter()
-    cent, labels, it, _ = oracle.ref_cluster(sub, init, gpu_iters, -1.0)
-    rep, _ = oracle.cc_uf(labels.reshape(c, c), w.connectivity)
-    oracle.compress(rep)
-    dt = time.perf_counter() - t0
+    def run(parallel):
+        t0 = time.perf_counter()
+        cent, labels, it, _ = oracle.ref_cluster(sub, init, gpu_iters, -1.0,
+                                                 parallel=parallel)
+        rep, _ = oracle.cc_uf(labels.reshape(c, c), w.connectivity)
+        oracle.compress(rep)
+        return time.perf_counter() - t0, it, labels
+
+    dt1, it, lab1 = run(False)
+    # all cores: the OpenMP build of the same oracle (per-pixel assignment
+    # split over OMP_NUM_THREADS threads; identical results)
+    dtp, _, labp = run(True)
+    cores = oracle.threads(True)
+    if not np.array_equal(lab1, labp):
+        raise RuntimeError("OpenMP oracle build disagrees with the plain build")
     return {
-        "value": round(c * c / dt / 1e6, 4),
+        "value": round(c * c / dtp / 1e6, 4),
         "unit": "Mpix/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
         "sample": (f"center {c}x{c} crop of the {w.name} image, k={w.k}, {it} Lloyd "
                    f"iterations of the reference algorithm (dense float distances, "
-                   f"sorted reduce_by_key centroids) + union-find CC "
-                   f"({w.connectivity}-conn) + sort/unique compress; oracle C, 1 thread; "
-                   f"{dt:.2f} s"),
+                   f"label-sorted reduce_by_key centroids) + union-find CC "
+                   f"({w.connectivity}-conn) + sort/unique compress; oracle C with OpenMP "
+                   f"over pixels, {cores} threads: {dtp:.2f} s; 1 thread: {dt1:.2f} s"),
+        "single_thread": {"value": round(c * c / dt1 / 1e6, 4), "cores": 1,
+                          "seconds": round(dt1, 2)},
     }
 
 

@@ -14,8 +14,12 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "_build" / "liboracle.so"
+# the same code built with OpenMP: only the CPU baseline's all-cores timing
+# loads it (set OMP_NUM_THREADS before the first call)
+LIB_OMP = HERE / "_build" / "liboracle_omp.so"
 
 _lib = None
+_lib_omp = None
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -30,6 +34,7 @@ _SIGS = {
     "oracle_cc_uf": (_I64, [_P, _I, _I, _I, _P]),
     "oracle_compress": (_I64, [_P, _I64]),
     "oracle_sample_indices": (None, [ctypes.c_uint64, _I64, _I, _P]),
+    "oracle_threads": (_I, []),
     "ref_label_points": (None, [_P, _I, _P, _I64, _P]),
     "ref_calculate_centroids": (None, [_P, _P, _I64, _P, _I]),
     "ref_cluster": (_I, [_P, _I64, _P, _I, _P, _I, _F, ctypes.POINTER(_F)]),
@@ -43,17 +48,25 @@ def build() -> Path:
     return LIB
 
 
-def lib() -> ctypes.CDLL:
-    global _lib
+def _load(path: Path) -> ctypes.CDLL:
+    if not path.exists():
+        build()
+    h = ctypes.CDLL(str(path))
+    for name, (res, args) in _SIGS.items():
+        f = getattr(h, name)
+        f.restype = res
+        f.argtypes = args
+    return h
+
+
+def lib(parallel: bool = False) -> ctypes.CDLL:
+    global _lib, _lib_omp
+    if parallel:
+        if _lib_omp is None:
+            _lib_omp = _load(LIB_OMP)
+        return _lib_omp
     if _lib is None:
-        if not LIB.exists():
-            build()
-        h = ctypes.CDLL(str(LIB))
-        for name, (res, args) in _SIGS.items():
-            f = getattr(h, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = h
+        _lib = _load(LIB)
     return _lib
 
 
@@ -137,13 +150,14 @@ def ref_calculate_centroids(labels, pts, k):
     return cent.reshape(3, k)
 
 
-def ref_cluster(pts, init, max_iter, threshold):
+def ref_cluster(pts, init, max_iter, threshold, parallel=False):
     pts = _f32(pts)
     cent = _f32(init).copy()
     labels = np.empty(pts.shape[1], np.int32)
     d = ctypes.c_float(0.0)
-    it = lib().ref_cluster(pts.ctypes.data, pts.shape[1], cent.ctypes.data, cent.shape[1],
-                           labels.ctypes.data, int(max_iter), float(threshold), ctypes.byref(d))
+    it = lib(parallel).ref_cluster(pts.ctypes.data, pts.shape[1], cent.ctypes.data,
+                                   cent.shape[1], labels.ctypes.data, int(max_iter),
+                                   float(threshold), ctypes.byref(d))
     return cent, labels, int(it), float(d.value)
 
 
@@ -165,3 +179,8 @@ def canonical_min(rep):
     mins = np.full(int(rep.max()) + 1 if n else 1, np.iinfo(np.int64).max, dtype=np.int64)
     np.minimum.at(mins, rep, idx)
     return mins[rep].astype(np.int64)
+
+
+def threads(parallel=False):
+    """Threads the (OpenMP) build runs its per-pixel loops on."""
+    return int(lib(parallel).oracle_threads())

@@ -12,6 +12,18 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads the OpenMP build's parallel loops use (1 in the plain build). */
+int oracle_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 /* ------------------------------------------------------------ FP64 ---- */
 
@@ -223,6 +235,8 @@ static float ref_dist(const float* pts, int64_t n, int64_t i, const float* cent,
 
 void ref_label_points(const float* cent, int k, const float* pts, int64_t n,
                       int32_t* labels) {
+  /* pixels are independent: the OpenMP build (timing only) splits them */
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     float best = INFINITY;
     int bj = 0;
@@ -252,11 +266,29 @@ static int cmp_keyed(const void* a, const void* b) {
 void ref_calculate_centroids(const int32_t* labels, const float* pts,
                              int64_t n, float* cent, int k) {
   keyed_t* s = (keyed_t*)malloc((size_t)n * sizeof(keyed_t));
-  for (int64_t i = 0; i < n; ++i) {
-    s[i].label = labels[i];
-    s[i].index = i;
+  /* sort_by_key of (label, index) pairs (centroids.cu:48): labels in
+   * [0, k) take a stable counting sort (the radix sort's order: by label,
+   * then index), anything else the comparison sort */
+  int in_range = 1;
+  for (int64_t i = 0; i < n; ++i)
+    if (labels[i] < 0 || labels[i] >= k) in_range = 0;
+  if (in_range) {
+    int64_t* start = (int64_t*)calloc((size_t)k + 1, sizeof(int64_t));
+    for (int64_t i = 0; i < n; ++i) ++start[labels[i] + 1];
+    for (int j = 0; j < k; ++j) start[j + 1] += start[j];
+    for (int64_t i = 0; i < n; ++i) {
+      keyed_t* e = &s[start[labels[i]]++];
+      e->label = labels[i];
+      e->index = i;
+    }
+    free(start);
+  } else {
+    for (int64_t i = 0; i < n; ++i) {
+      s[i].label = labels[i];
+      s[i].index = i;
+    }
+    qsort(s, (size_t)n, sizeof(keyed_t), cmp_keyed);
   }
-  qsort(s, (size_t)n, sizeof(keyed_t), cmp_keyed);
   /* upper_bound cumulative counts, stored as float (centroids.cu:53-57) */
   float* rval = (float*)malloc((size_t)k * sizeof(float));
   int64_t pos = 0;

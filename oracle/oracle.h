@@ -60,6 +60,7 @@ int64_t oracle_compress(int32_t* labels, int64_t n);
 
 /* SplitMix64 sampler: k distinct indices in [0, n) (index = high 64 bits of
  * draw * n, redrawn on a repeat), as split_hip_uniform_random_initialize. */
+int oracle_threads(void);
 void oracle_sample_indices(uint64_t seed, int64_t n, int k, int64_t* idx);
 
 /* ---- restatement of the reference (float, quirks included) ---- */
