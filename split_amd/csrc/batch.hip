@@ -118,6 +118,54 @@ __device__ __forceinline__ int find_image(const BatchImg* img, int count,
   return lo;
 }
 
+// The fields of one image the pixel-space kernels use, held in registers
+// while a wave walks that image.
+struct PxImg {
+  int32_t* labels;
+  int32_t* segments;
+  unsigned* bitmap;
+  unsigned* prefix;
+  const uint32_t* pstate;
+  int64_t n, px0;
+  int W, H, iter, bounded;
+};
+
+__device__ __forceinline__ PxImg px_image(const BatchImg& m) {
+  return PxImg{m.labels, m.segments, m.bitmap, m.prefix, m.pstate,
+               (int64_t)m.W * m.H, m.px0, m.W, m.H, m.iter, m.bounded};
+}
+
+// A wave's contiguous share of the padded global pixel space in 64-pixel
+// units (each image starts at a multiple of 64, so a unit lies in one
+// image): calls f(m, q0) with q0 the unit's first pixel in image m (lane l
+// owns pixel q0 + l, which exists while q0 + l < m.n).  The image is found
+// by one binary search, then by walking forward; a grid-stride loop paid a
+// binary search (8 dependent table loads) per 64 pixels.
+template <class F>
+__device__ __forceinline__ void for_each_unit(const BatchArgs& a, F&& f) {
+  const int64_t units = a.px >> 6;
+  const int wpb = (int)(blockDim.x >> 6);
+  const int64_t nw = (int64_t)gridDim.x * wpb;
+  const int64_t gw = (int64_t)blockIdx.x * wpb +
+                     __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t u0 = units * gw / nw, u1 = units * (gw + 1) / nw;
+  if (u0 >= u1) return;
+  int i = find_image<&BatchImg::px0>(a.img, a.count, u0 << 6);
+  PxImg m = px_image(a.img[i]);
+  int64_t next = i + 1 < a.count ? a.img[i + 1].px0 : INT64_MAX;
+  for (int64_t u = u0; u < u1; ++u) {
+    const int64_t g0 = u << 6;
+    if (g0 >= next) {
+      while (g0 >= next) {
+        ++i;
+        next = i + 1 < a.count ? a.img[i + 1].px0 : INT64_MAX;
+      }
+      m = px_image(a.img[i]);
+    }
+    f(m, g0 - m.px0);
+  }
+}
+
 __device__ __forceinline__ Geometry image_geometry(const BatchImg& m) {
   Geometry g{};
   g.pts = m.pts;
@@ -471,22 +519,13 @@ __global__ __launch_bounds__(256) void batch_labels_plain(Group g) {
 
 // Labels of the bounded images, from the per-pixel state (images that ran
 // an iteration).
-__global__ __launch_bounds__(256) void batch_labels_state(Group g) {
+__global__ __launch_bounds__(256) void batch_labels_state(BatchArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
-  for (int64_t s = g.t0 + gw; s < g.t1; s += nw) {
-    const int i = find_image<&BatchImg::tile0>(g.img, g.count, s);
-    const BatchImg& m = g.img[i];
-    if (m.iter == 0) continue;
-    const Geometry geo = image_geometry(m);
-    int64_t idx[4];
-    bool v[4], lvec;
-    segment_index<kTile>(geo, s - m.tile0, lane, idx, v, lvec);
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (v[j]) m.labels[idx[j]] = (int32_t)(m.pstate[idx[j]] & 0xFFu);
-  }
+  for_each_unit(a, [&](const PxImg& m, int64_t q0) {
+    const int64_t q = q0 + lane;
+    if (m.bounded && m.iter > 0 && q < m.n)
+      m.labels[q] = (int32_t)(m.pstate[q] & 0xFFu);
+  });
 }
 
 __global__ void batch_finish(BatchArgs a) {
@@ -524,15 +563,10 @@ __global__ __launch_bounds__(3 * kCcTile) void batch_ccl_merge(BatchArgs a) {
 // Root jumps (ccl_jump_one) over the padded global pixel space.
 __global__ __launch_bounds__(256) void batch_ccl_jump(BatchArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.px;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t wave0 = __builtin_amdgcn_readfirstlane(
-        (int)((base + threadIdx.x - lane) >> 6)) * (int64_t)64;
-    const int i = find_image<&BatchImg::px0>(a.img, a.count, wave0);
-    const BatchImg& m = a.img[i];
-    const int64_t q = wave0 - m.px0 + lane;
-    if (q < (int64_t)m.W * m.H) ccl_jump_one(m.segments, q, m.W);
-  }
+  for_each_unit(a, [&](const PxImg& m, int64_t q0) {
+    const int64_t q = q0 + lane;
+    if (q < m.n) ccl_jump_one(m.segments, q, m.W);
+  });
 }
 
 // Check of the flattened forests (ccl_check_one) over the padded global
@@ -540,41 +574,26 @@ __global__ __launch_bounds__(256) void batch_ccl_jump(BatchArgs a) {
 template <int CONN>
 __global__ __launch_bounds__(256) void batch_ccl_check(BatchArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.px;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t wave0 = __builtin_amdgcn_readfirstlane(
-        (int)((base + threadIdx.x - lane) >> 6)) * (int64_t)64;
-    const int i = find_image<&BatchImg::px0>(a.img, a.count, wave0);
-    const BatchImg& m = a.img[i];
-    const int64_t q = wave0 - m.px0 + lane;
-    if (q < (int64_t)m.W * m.H)
-      ccl_check_one<CONN>(m.labels, m.W, m.H, m.segments, q, a.ccl_flag);
-  }
+  for_each_unit(a, [&](const PxImg& m, int64_t q0) {
+    const int64_t q = q0 + lane;
+    if (q < m.n) ccl_check_one<CONN>(m.labels, m.W, m.H, m.segments, q, a.ccl_flag);
+  });
 }
 
-// Flatten + root bitmap over the padded global pixel space (a wave never
-// spans two images, as each image starts at a multiple of 64).  recheck: run
+// Flatten + root bitmap over the padded global pixel space.  recheck: run
 // only when the check pass found a failure.
 __global__ __launch_bounds__(256) void batch_flatten(BatchArgs a, int compress,
                                                      int recheck) {
   if (recheck && *a.ccl_flag == 0u) return;
   const int lane = threadIdx.x & (kWave - 1);
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.px;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = base + threadIdx.x;
-    const int64_t wave0 = __builtin_amdgcn_readfirstlane(
-        (int)((base + threadIdx.x - lane) >> 6)) * (int64_t)64;
-    const int i = find_image<&BatchImg::px0>(a.img, a.count, wave0);
-    const BatchImg& m = a.img[i];
-    const int64_t n = (int64_t)m.W * m.H;
-    const int64_t q = p - m.px0;
-    const bool root = q < n && ccl_flatten_one(m.segments, q) == q;
+  for_each_unit(a, [&](const PxImg& m, int64_t q0) {
+    const int64_t q = q0 + lane;
+    const bool root = q < m.n && ccl_flatten_one(m.segments, q) == q;
     const uint64_t bits = ballot(root);
-    if (compress && lane == 0 && wave0 - m.px0 < n) {
-      *reinterpret_cast<uint2*>(m.bitmap + ((wave0 - m.px0) >> 5)) =
+    if (compress && lane == 0 && q0 < m.n)
+      *reinterpret_cast<uint2*>(m.bitmap + (q0 >> 5)) =
           make_uint2((uint32_t)bits, (uint32_t)(bits >> 32));
-    }
-  }
+  });
 }
 
 // Exclusive popcount prefix of each image's root bitmap, one block per image.
@@ -606,21 +625,15 @@ __global__ __launch_bounds__(1024) void batch_scan(BatchArgs a) {
 
 __global__ __launch_bounds__(256) void batch_map(BatchArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.px;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = base + threadIdx.x;
-    const int64_t wave0 = __builtin_amdgcn_readfirstlane(
-        (int)((base + threadIdx.x - lane) >> 6)) * (int64_t)64;
-    const int i = find_image<&BatchImg::px0>(a.img, a.count, wave0);
-    const BatchImg& m = a.img[i];
-    const int64_t q = p - m.px0;
-    if (q < (int64_t)m.W * m.H) {
+  for_each_unit(a, [&](const PxImg& m, int64_t q0) {
+    const int64_t q = q0 + lane;
+    if (q < m.n) {
       const uint32_t v = (uint32_t)m.segments[q];
       const uint32_t wd = v >> 5;
       m.segments[q] =
           (int32_t)(m.prefix[wd] + __popc(m.bitmap[wd] & ((1u << (v & 31)) - 1u)));
     }
-  }
+  });
 }
 
 // ------------------------------------------------------------------ host ---
@@ -825,7 +838,7 @@ static void enqueue_batch(const BatchArgs& a, const Groups& G, int conn,
                        G.plain);
   if (G.bounded.count)
     hipLaunchKernelGGL(batch_labels_state, dim3(8 * cus), dim3(256), 0, stream,
-                       G.bounded);
+                       a);
   hipLaunchKernelGGL(batch_finish, dim3(a.count), dim3(256), 0, stream, a);
   const int cgrid = (int)std::min<int64_t>(a.ctiles, (int64_t)cus * 16);
   if (conn == 8) {
