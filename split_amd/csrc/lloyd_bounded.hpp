@@ -254,39 +254,53 @@ __device__ __forceinline__ void walk_tiles(
     }
   } else {
     static_assert(MODE == kTile || MODE == kTileVec, "bounded walks cover tiles");
-    // Tile s wholly inside the image (wave-uniform); o = offset of this
-    // lane's first pixel (tile indices of one image fit in int).
-    auto tile_inner = [&](int64_t s, int64_t& o) {
-      const int t = (int)s;
-      const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
-      o = (int64_t)(ty * kTileSide + (lane >> 2)) * a.width + tx * kTileSide +
-          4 * (lane & 3);
-      return (ty + 1) * kTileSide <= a.height && (tx + 1) * kTileSide <= a.width;
+    // Tile position (column tx, row ty) of tile s, wave-uniform: advanced by
+    // one per tile (the walks take consecutive tiles), divided out only when
+    // a walk jumps.  Tile indices of one image fit in int.
+    struct Pos {
+      int64_t s;
+      int tx, ty;
+    };
+    auto locate = [&](int64_t t, Pos& p) {
+      if (t == p.s + 1) {
+        p.s = t;
+        if (++p.tx == a.tiles_x) {
+          p.tx = 0;
+          ++p.ty;
+        }
+      } else {
+        p.s = t;
+        p.ty = (int)t / a.tiles_x;
+        p.tx = (int)t - p.ty * a.tiles_x;
+      }
     };
     // One 16-byte load per lane on every path, so that the wait for a
     // tile's state (issued a tile earlier) never covers the next tile's
     // load: lanes below the image read the tile's first row, lanes past its
     // right edge read on into the next row or the kStatePad entries after
     // the last pixel; visit() masks such pixels.
-    auto load_state = [&](int64_t s, bool have, uint32_t (&ps)[4]) {
-      const int t = have ? (int)s : 0;
-      const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
-      const int row = ty * kTileSide + (lane >> 2);
+    auto load_state = [&](const Pos& p, bool have, uint32_t (&ps)[4]) {
+      const int row = p.ty * kTileSide + (lane >> 2);
       const int64_t o =
-          (int64_t)(row < a.height ? row : ty * kTileSide) * a.width +
-          tx * kTileSide + 4 * (lane & 3);
+          have ? (int64_t)(row < a.height ? row : p.ty * kTileSide) * a.width +
+                     p.tx * kTileSide + 4 * (lane & 3)
+               : 0;
       const U4 q = *gmem(reinterpret_cast<const U4*>(a.pstate + o));
       ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
     };
-    auto visit = [&](int64_t s, const uint32_t (&ps)[4]) {
+    auto visit = [&](const Pos& p, const uint32_t (&ps)[4]) {
       int64_t idx[4];
       bool v[4], lvec;
       int old[4];
       bool fail[4];
       uint64_t fb[4];
       int nfail = 0;
-      int64_t o;
-      if (tile_inner(s, o)) {  // every pixel exists: no per-lane guards
+      const int row = p.ty * kTileSide + (lane >> 2);
+      const int col = p.tx * kTileSide + 4 * (lane & 3);
+      const int64_t o = (int64_t)row * a.width + col;
+      if ((p.ty + 1) * kTileSide <= a.height &&
+          (p.tx + 1) * kTileSide <= a.width) {
+        // every pixel exists: no per-lane guards
         lvec = MODE == kTileVec;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -298,9 +312,12 @@ __device__ __forceinline__ void walk_tiles(
           nfail += __builtin_popcountll(fb[i]);
         }
       } else {
-        segment_index<MODE>(a, s, lane, idx, v, lvec);
+        const bool rowok = row < a.height;
+        lvec = MODE == kTileVec && rowok && col + 3 < a.width;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
+          idx[i] = o + i;
+          v[i] = rowok && col + i < a.width;
           old[i] = v[i] ? (int)(ps[i] & 0xFFu) : -1;
           fail[i] = v[i] &&
                     !(state_budget(ps[i]) - fl[v[i] ? old[i] : 0] > margin);
@@ -328,17 +345,21 @@ __device__ __forceinline__ void walk_tiles(
       }
       if (qlen >= 256) drain();
     };
+    Pos pa{-2, 0, 0};
     int64_t s = 0;
     bool have = next(s);
+    if (have) locate(s, pa);
     uint32_t ps[4];
-    load_state(s, have, ps);
+    load_state(pa, have, ps);
     while (have) {
       int64_t s2 = 0;
       const bool have2 = next(s2);
+      Pos pb = pa;
+      if (have2) locate(s2, pb);
       uint32_t ps2[4];
-      load_state(s2, have2, ps2);
-      visit(s, ps);
-      s = s2;
+      load_state(pb, have2, ps2);
+      visit(pa, ps);
+      pa = pb;
       have = have2;
 #pragma unroll
       for (int i = 0; i < 4; ++i) ps[i] = ps2[i];
