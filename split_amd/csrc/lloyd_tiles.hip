@@ -577,25 +577,27 @@ __global__ __launch_bounds__(256) void tiles_absmax(const float* pts,
 __global__ __launch_bounds__(256) void tiles_absmax_pack(const float* pts,
                                                          int64_t n, F4* aos,
                                                          unsigned* out) {
+  // one pixel per lane and step: each plane load reads 256 contiguous bytes
+  // and each store writes 1 KiB, four steps in flight
   float m = 0.f;
-  const int64_t quads = n / 4;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < quads;
-       q += stride) {
-    const int64_t i = 4 * q;
-    const F4 X = *reinterpret_cast<const F4*>(pts + i);
-    const F4 Y = *reinterpret_cast<const F4*>(pts + n + i);
-    const F4 Z = *reinterpret_cast<const F4*>(pts + 2 * n + i);
-    aos[i] = F4{X.x, Y.x, Z.x, 0.f};
-    aos[i + 1] = F4{X.y, Y.y, Z.y, 0.f};
-    aos[i + 2] = F4{X.z, Y.z, Z.z, 0.f};
-    aos[i + 3] = F4{X.w, Y.w, Z.w, 0.f};
-    m = fmaxf(m, fmaxf(fmaxf(fmaxf(fabsf(X.x), fabsf(X.y)), fmaxf(fabsf(X.z), fabsf(X.w))),
-                       fmaxf(fmaxf(fmaxf(fabsf(Y.x), fabsf(Y.y)), fmaxf(fabsf(Y.z), fabsf(Y.w))),
-                             fmaxf(fmaxf(fabsf(Z.x), fabsf(Z.y)), fmaxf(fabsf(Z.z), fabsf(Z.w))))));
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    float x[4], y[4], z[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t j = i + u * stride;
+      x[u] = pts[j];
+      y[u] = pts[n + j];
+      z[u] = pts[2 * n + j];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      aos[i + u * stride] = F4{x[u], y[u], z[u], 0.f};
+      m = fmaxf(m, fmaxf(fabsf(x[u]), fmaxf(fabsf(y[u]), fabsf(z[u]))));
+    }
   }
-  for (int64_t i = 4 * quads + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-       i < n; i += stride) {
+  for (; i < n; i += stride) {
     const float x = pts[i], y = pts[n + i], z = pts[2 * n + i];
     aos[i] = F4{x, y, z, 0.f};
     m = fmaxf(m, fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z))));
