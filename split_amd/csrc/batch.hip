@@ -70,7 +70,7 @@ struct BatchImg {
   // device state
   int cur, iter, done, nseg;
   float shift;
-  unsigned maxabs;
+  unsigned maxabs[3];  // largest |v| of each channel (float bits)
   int tiles_done;
   int bounded;  // iterations after the first are bounded (keep the totals)
 };
@@ -188,15 +188,16 @@ __global__ void batch_init(BatchArgs a) {
     m.done = a.max_iter <= 0;
     m.nseg = -1;
     m.shift = 0.f;
-    m.maxabs = 0u;
     m.tiles_done = 0;
   }
+  if (threadIdx.x < 3) m.maxabs[threadIdx.x] = 0u;
   for (int i = threadIdx.x; i < k; i += blockDim.x) m.fcum[i] = 0.f;
 }
 
-// Largest |v| per image.  A wave takes a contiguous run of the batch's
-// Lloyd tiles and keeps a running maximum while the run stays in one image,
-// so an image receives one atomic per wave that touches it.
+// Largest |v| of each channel per image.  A wave takes a contiguous run of
+// the batch's Lloyd tiles and keeps running maxima while the run stays in
+// one image, so an image receives one atomic per channel and wave that
+// touches it.
 __global__ __launch_bounds__(256) void batch_absmax(BatchArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(256) void batch_absmax(BatchArgs a) {
     const Geometry g = image_geometry(m);
     const int64_t end = std::min<int64_t>(s1, m.tile0 + g.nseg);
     const bool vec = aligned4(g.pts);  // dword-aligned 16-byte loads
-    float mx = 0.f;
+    float mx = 0.f, my = 0.f, mz = 0.f;
     for (; s < end; ++s) {
       int64_t idx[4];
       bool v[4], lvec;
@@ -218,11 +219,20 @@ __global__ __launch_bounds__(256) void batch_absmax(BatchArgs a) {
       float x[4], y[4], z[4];
       load_colours(g, idx, v, lvec, x, y, z);
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        mx = v[j] ? fmaxf(mx, fmaxf(fabsf(x[j]), fmaxf(fabsf(y[j]), fabsf(z[j])))) : mx;
+      for (int j = 0; j < 4; ++j) {
+        mx = v[j] ? fmaxf(mx, fabsf(x[j])) : mx;
+        my = v[j] ? fmaxf(my, fabsf(y[j])) : my;
+        mz = v[j] ? fmaxf(mz, fabsf(z[j])) : mz;
+      }
     }
     mx = wave_allreduce(mx, OpMax());
-    if (lane == 0) atomicMax(&a.img[i].maxabs, __float_as_uint(mx));
+    my = wave_allreduce(my, OpMax());
+    mz = wave_allreduce(mz, OpMax());
+    if (lane == 0) {
+      atomicMax(&a.img[i].maxabs[0], __float_as_uint(mx));
+      atomicMax(&a.img[i].maxabs[1], __float_as_uint(my));
+      atomicMax(&a.img[i].maxabs[2], __float_as_uint(mz));
+    }
   }
 }
 
@@ -265,8 +275,8 @@ __device__ void batch_finish_segment(const BatchArgs& a, BatchImg& m,
   if (!last) return;
   __threadfence();
   // this wave finalises image i for this iteration
-  const float fscale = scale_of(m.maxabs);
-  const double inv = 1.0 / (double)fscale;
+  const QScale qs = qscale_of(m.maxabs);
+  const float sc[3] = {qs.x, qs.y, qs.z};
   const float* cent = m.cbuf + (size_t)m.cur * 3 * k;
   float* cent_out = m.cbuf + (size_t)(1 - m.cur) * 3 * k;
   double d2 = 0.0;
@@ -279,7 +289,7 @@ __device__ void batch_finish_segment(const BatchArgs& a, BatchImg& m,
       const long long sum = __hip_atomic_load(&m.totals[c * k + lane],
                                               __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
-      const float nc = cnt > 0 ? (float)((double)sum * inv / (double)cnt) : 0.f;
+      const float nc = dequantise_mean(sum, cnt, sc[c]);
       const double d = (double)nc - (double)cent[c * k + lane];
       d2 += d * d;
       cent_out[c * k + lane] = nc;
@@ -355,7 +365,7 @@ __global__ __launch_bounds__(1024) void batch_step_plain(BatchArgs a, Group g) {
       cz[0] = live ? cent[2 * k + lane] : 0.f;
       cn[0] = fmaf(cz[0], cz[0], fmaf(cy[0], cy[0], cx[0] * cx[0]));
     }
-    const float fscale = scale_of(m.maxabs);
+    const QScale qs = qscale_of(m.maxabs);
     const bool vec = aligned4(geo.pts);  // dword-aligned 16-byte loads
     IRun run{0, 0, 0, 0, -1, 0};
     for (int64_t t = s; t < end; ++t) {
@@ -371,9 +381,9 @@ __global__ __launch_bounds__(1024) void batch_step_plain(BatchArgs a, Group g) {
       int qx[4], qy[4], qz[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        qx[j] = (int)(x[j] * fscale);
-        qy[j] = (int)(y[j] * fscale);
-        qz[j] = (int)(z[j] * fscale);
+        qx[j] = quantise(x[j], qs.x);
+        qy[j] = quantise(y[j], qs.y);
+        qz[j] = quantise(z[j], qs.z);
       }
       const int l0 = readfirstlane_i(lab[0]);
       const bool same = lab[0] == l0 && lab[1] == l0 && lab[2] == l0 &&
@@ -431,8 +441,8 @@ __device__ __forceinline__ void bounded_run(const BatchArgs& a, const Group& g, 
       cz[0] = live ? cent[2 * k + lane] : 0.f;
       cn[0] = fmaf(cz[0], cz[0], fmaf(cy[0], cy[0], cx[0] * cx[0]));
     }
-    const float maxabs = __uint_as_float(m.maxabs);
-    const float fscale = scale_of(m.maxabs);
+    const float maxabs = qmax_of(m.maxabs);
+    const QScale qs = qscale_of(m.maxabs);
     const float margin = 1e-5f * (1.f + 4.f * maxabs);
     if (lane < k) w.fl[lane] = m.fcum[lane];
     __builtin_amdgcn_wave_barrier();
@@ -441,11 +451,11 @@ __device__ __forceinline__ void bounded_run(const BatchArgs& a, const Group& g, 
     const int64_t t0 = s - m.tile0, t1 = end - m.tile0;
     if (vec)
       walk_tiles<1, kTileVec, FIRST>(ta, TileRange{t0, t1}, cx, cy, cz,
-                                     cn, w.fl, fscale, margin, w.slice,
+                                     cn, w.fl, qs, margin, w.slice,
                                      w.qidx, w.qold, lane);
     else
       walk_tiles<1, kTile, FIRST>(ta, TileRange{t0, t1}, cx, cy, cz, cn,
-                                  w.fl, fscale, margin, w.slice, w.qidx,
+                                  w.fl, qs, margin, w.slice, w.qidx,
                                   w.qold, lane);
     batch_finish_segment(a, m, (int)(end - s), w, lane);
     s = end;
@@ -875,10 +885,14 @@ struct GraphEntry {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
-// (device, temp, images, plain images, plain tiles, tiles, pixels, max_iter,
-//  threshold, connectivity, compress, tuned plain / bounded shapes)
+// Everything a captured graph bakes in: (device, temp, images, plain
+// images, plain tiles, Lloyd tiles, CC tiles, pixels, max_iter, threshold,
+// connectivity, compress, tuned plain / bounded shapes).  Per-image sizes,
+// k and buffers are read from the descriptor table at run time; the totals
+// are launch bounds and grid sizes.
 using GraphKey = std::tuple<int, void*, int, int, int64_t, int64_t, int64_t,
-                            int, float, int, int, int, int, int, int>;
+                            int64_t, int, float, int, int, int, int, int,
+                            int>;
 static std::mutex g_graph_mu;
 static std::map<GraphKey, GraphEntry> g_graphs;
 
@@ -938,10 +952,10 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
     SPLIT_HIP_TRY(autotune_group(a, G.bounded, true, kb, device, stream, &G.tb));
 
   if (use_graph) {
-    const GraphKey key{device, temp, count, P.nplain, P.tiles_plain, P.tiles,
-                       P.px, a.max_iter, threshold, connectivity,
-                       compress != 0, G.tp.threads, G.tp.per_cu,
-                       G.tb.threads, G.tb.per_cu};
+    const GraphKey key{device,      temp,          count,      P.nplain,
+                       P.tiles_plain, P.tiles,     P.ctiles,   P.px,
+                       a.max_iter,  threshold,     connectivity, compress != 0,
+                       G.tp.threads, G.tp.per_cu,  G.tb.threads, G.tb.per_cu};
     hipGraphExec_t exec = nullptr;
     {
       std::lock_guard<std::mutex> lock(g_graph_mu);

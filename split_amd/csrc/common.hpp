@@ -191,9 +191,9 @@ struct DriverRes {
 int driver_res(DriverRes** out);
 
 // Bounded tiled Lloyd (lloyd_tiles.hip).
-// Largest |v| of count floats as float bits into *out (which must start at
-// 0), stream-ordered.
-int launch_absmax(const float* pts, int64_t count, unsigned* out,
+// Largest |v| of each of the three planes of n floats as float bits into
+// out[0..3) (which must start at 0), stream-ordered.
+int launch_absmax(const float* pts, int64_t n, unsigned* out,
                   hipStream_t stream);
 // calculate_centroids for any k (segment means; exact integer sums).
 size_t segment_means_workspace(int64_t k);

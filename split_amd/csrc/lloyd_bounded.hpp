@@ -21,7 +21,7 @@ struct TileArgs : Geometry {
   long long* partials;        // step accumulator copies, [copies][4k]
   float* fcum;                // per-centroid bound consumption F[j], k
   uint32_t* pstate;           // per-pixel state (see pack_state)
-  const unsigned* maxabs;     // bits of max |v| (non-negative float)
+  const unsigned* maxabs;     // bits of max |v| of each channel (3)
   LloydState* state;
   Mailbox* mbox;
   // bounded iterations of the single-image path: units (pairs of adjacent
@@ -76,7 +76,7 @@ constexpr int kAtomicChanges = 16;
 // slack, writes their state and accumulates.
 template <bool FULL>
 __device__ __forceinline__ void commit_assignment(
-    const TileArgs& a, const float* fl, float fscale, int lane,
+    const TileArgs& a, const float* fl, QScale qs, int lane,
     const int64_t (&idx)[4], const bool (&v)[4], bool lvec,
     const int (&old)[4], const float (&x)[4], const float (&y)[4],
     const float (&z)[4], const int (&lab)[4], const float (&slack)[4],
@@ -98,9 +98,9 @@ __device__ __forceinline__ void commit_assignment(
   int qx[4], qy[4], qz[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    qx[i] = (int)(x[i] * fscale);
-    qy[i] = (int)(y[i] * fscale);
-    qz[i] = (int)(z[i] * fscale);
+    qx[i] = quantise(x[i], qs.x);
+    qy[i] = quantise(y[i], qs.y);
+    qz[i] = quantise(z[i], qs.z);
   }
   if constexpr (FULL) {
     const int l0 = readfirstlane_i(lab[0]);
@@ -167,14 +167,14 @@ template <int CPL, bool FULL>
 __device__ __forceinline__ void reassign(
     const TileArgs& a, const float (&cx)[CPL], const float (&cy)[CPL],
     const float (&cz)[CPL], const float (&cn)[CPL], const float* fl,
-    float fscale, int lane, const int64_t (&idx)[4], const bool (&v)[4],
+    QScale qs, int lane, const int64_t (&idx)[4], const bool (&v)[4],
     bool lvec, const int (&old)[4], long long* slice, IRun& run) {
   float x[4], y[4], z[4];
   load_colours(a, idx, v, lvec, x, y, z);
   int lab[4];
   float slack[4];
   assign_segment<CPL, true>(cx, cy, cz, cn, a.k, lane, x, y, z, v, lab, slack);
-  commit_assignment<FULL>(a, fl, fscale, lane, idx, v, lvec, old, x, y, z,
+  commit_assignment<FULL>(a, fl, qs, lane, idx, v, lvec, old, x, y, z,
                           lab, slack, slice, run);
 }
 
@@ -210,7 +210,7 @@ template <int CPL, int MODE, bool FULL, class NextTile>
 __device__ __forceinline__ void walk_tiles(
     const TileArgs& a, NextTile next, const float (&cx)[CPL],
     const float (&cy)[CPL], const float (&cz)[CPL], const float (&cn)[CPL],
-    const float* fl, float fscale, float margin, long long* slice, int* qidx,
+    const float* fl, QScale qs, float margin, long long* slice, int* qidx,
     int* qold, int lane) {
   const int k = a.k;
   IRun run{0, 0, 0, 0, -1, 0};
@@ -229,7 +229,7 @@ __device__ __forceinline__ void walk_tiles(
       idx[i] = qidx[v[i] ? e : 0];
       old[i] = v[i] ? qold[e] : -1;
     }
-    reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, false,
+    reassign<CPL, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v, false,
                          old, slice, run);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -249,7 +249,7 @@ __device__ __forceinline__ void walk_tiles(
       bool v[4], lvec;
       segment_index<MODE>(a, s, lane, idx, v, lvec);
       const int old[4] = {-1, -1, -1, -1};
-      reassign<CPL, true>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, lvec,
+      reassign<CPL, true>(a, cx, cy, cz, cn, fl, qs, lane, idx, v, lvec,
                           old, slice, run);
     }
   } else {
@@ -327,7 +327,7 @@ __device__ __forceinline__ void walk_tiles(
       }
       if (nfail == 0) return;  // every pixel provably unchanged
       if (nfail > kWholeTileFails) {
-        reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v,
+        reassign<CPL, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v,
                              lvec, old, slice, run);
         return;
       }

@@ -1,11 +1,14 @@
 // Exact integer accumulation of quantised pixel sums, shared by the
 // single-image and the batched Lloyd kernels.
 //
-// Channels are quantised once per image, q = trunc(v * 2^S) with
-// |q| <= 2^22 (S from the image's largest |v|), and summed as integers:
-// per-lane int32 registers, exact wave sums, int64 LDS slices and int64
-// totals.  Integer sums do not depend on order, so every traversal of an
-// image gives the same totals.
+// Each channel c is quantised once per image with its own power-of-two
+// scale, q = rint(v * s_c) with |q| <= 2^23 - 1 (s_c from the channel's
+// largest |v|, see scale_of), and summed as integers: per-lane int32
+// registers, exact wave sums, int64 LDS slices and int64 totals.  Integer
+// sums do not depend on order, so every traversal of an image gives the same
+// totals.  Rounding to nearest moves each pixel by at most half a quantum,
+// so a mean moves by at most 1 / (2 s_c) <= max|v_c| * 2^-23, whatever the
+// scales of the other channels.
 #pragma once
 
 #include "common.hpp"
@@ -39,7 +42,7 @@ __device__ __forceinline__ void slice_add64(long long* slice, int k, int label,
 
 // Register run of a wave-uniform label: per-lane int32 sums of quantised
 // pixels, flushed (exact wide wave sums) on a label change or every
-// kRunFlushTiles tiles, which bounds a lane's sum by 2^30.
+// kRunFlushTiles tiles, which bounds a lane's sum by 256 (2^23 - 1) < 2^31.
 struct IRun {
   int x, y, z, n;
   int label;
@@ -61,7 +64,8 @@ __device__ __forceinline__ void irun_flush(IRun& r, long long* slice, int k,
 }
 
 // Adds sign * (q, 1) of each pixel with lab[i] >= 0 into its label's slice
-// entries, one distinct label at a time (|per-lane sum| <= 2^24).
+// entries, one distinct label at a time (|per-lane sum| <= 4 (2^23 - 1), so
+// a wave sum stays below 2^31).
 __device__ __forceinline__ void add_by_label(long long* slice, int k, int lane,
                                              const int (&lab)[4],
                                              const int (&qx)[4],
@@ -95,10 +99,41 @@ __device__ __forceinline__ void add_by_label(long long* slice, int k, int lane,
   }
 }
 
+// Quantisation scale of a channel whose largest |v| has the float bits
+// maxabs_bits: s = 2^(23 - e) with max|v| < 2^e, halved in the one case
+// where rint(max|v| * s) would reach 2^23, so |rint(v * s)| <= 2^23 - 1 for
+// every v of the channel.  v * s is exact (a power-of-two scaling).  e is
+// held at -100 or above so that s stays finite for channels of tiny or zero
+// range (an all-zero channel quantises to 0).
 __device__ __forceinline__ float scale_of(unsigned maxabs_bits) {
+  const float m = __uint_as_float(maxabs_bits);
   int e = 0;
-  frexpf(__uint_as_float(maxabs_bits), &e);  // max|v| <= 2^e (0 -> e = 0)
-  return ldexpf(1.f, 22 - e);
+  frexpf(m, &e);  // m < 2^e (m = 0 -> e = 0)
+  e = e < -100 ? -100 : e;
+  float s = ldexpf(1.f, 23 - e);
+  if (m * s > 8388607.f) s *= 0.5f;
+  return s;
+}
+
+// Per-channel scales of an image (maxabs: the three channels' largest |v|
+// as float bits).
+struct QScale {
+  float x, y, z;
+};
+__device__ __forceinline__ QScale qscale_of(const unsigned* maxabs) {
+  return QScale{scale_of(maxabs[0]), scale_of(maxabs[1]), scale_of(maxabs[2])};
+}
+__device__ __forceinline__ float qmax_of(const unsigned* maxabs) {
+  return fmaxf(__uint_as_float(maxabs[0]),
+               fmaxf(__uint_as_float(maxabs[1]), __uint_as_float(maxabs[2])));
+}
+__device__ __forceinline__ int quantise(float v, float s) {
+  return (int)rintf(v * s);
+}
+// Mean of a channel from its integer sum: sum / (count * s), in double.
+__device__ __forceinline__ float dequantise_mean(long long sum, long long cnt,
+                                                 float s) {
+  return cnt > 0 ? (float)((double)sum / ((double)cnt * (double)s)) : 0.f;
 }
 
 }  // namespace splithip

@@ -18,13 +18,14 @@
 // re-assign tiles with many failing pixels whole, and queue the other
 // failing pixels for re-assignment in batches of 256 (writing fresh state).
 //
-// Sums: channels are quantised once, q = trunc(v * 2^S) with |q| <= 2^22
-// (S from the largest |v| of the image), and summed as integers into int64
-// totals in temp: iteration 1 adds every pixel, later iterations add +q to
-// the new and -q to the old cluster of each pixel whose label changed.
-// Integer sums are exact, so totals are independent of order and of which
-// pixels were skipped, and identical run to run.  The quantisation moves a
-// mean by at most max|v| * 2^-22.
+// Sums: each channel is quantised once with its own power-of-two scale,
+// q = rint(v * s_c) with |q| <= 2^23 - 1 (s_c from the channel's largest
+// |v|), and summed as integers into int64 totals in temp: iteration 1 adds
+// every pixel, later iterations add +q to the new and -q to the old cluster
+// of each pixel whose label changed.  Integer sums are exact, so totals are
+// independent of order and of which pixels were skipped, and identical run
+// to run.  The quantisation moves a mean by at most max|v_c| * 2^-23 in
+// channel c.
 #include <algorithm>
 
 #include "common.hpp"
@@ -57,7 +58,7 @@ __device__ __forceinline__ long long* acc_buf(const TileArgs& a, int j) {
 // LDS scratch: tot (4k int64), d2 (k doubles), cs (3k floats: c_s on
 // return); fs (k floats) receives F_s.  Returns the stop decision (the same
 // in every block).
-__device__ bool tiles_prologue(const TileArgs& a, int s, float fscale,
+__device__ bool tiles_prologue(const TileArgs& a, int s, QScale qs,
                                long long* tot, double* d2, float* cs,
                                float* fs) {
   const int k = a.k, V = 4 * k;
@@ -77,14 +78,13 @@ __device__ bool tiles_prologue(const TileArgs& a, int s, float fscale,
   }
   __syncthreads();
   const float* cold = a.cbuf + (size_t)((s - 1) & 1) * 3 * k;
-  const double inv = 1.0 / (double)fscale;
+  const float sc[3] = {qs.x, qs.y, qs.z};
   if (tid < k) {
     const long long cnt = tot[3 * k + tid];
     double sq = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const long long sum = tot[c * k + tid];
-      const float nc = cnt > 0 ? (float)((double)sum * inv / (double)cnt) : 0.f;
+      const float nc = dequantise_mean(tot[c * k + tid], cnt, sc[c]);
       const double d = (double)nc - (double)cold[c * k + tid];
       sq += d * d;
       cs[c * k + tid] = nc;
@@ -202,7 +202,8 @@ struct BoundedWalker {
   const float (&cz)[CPL];
   const float (&cn)[CPL];
   const float* fl;
-  float fscale, margin;
+  QScale qs;
+  float margin;
   long long* slice;
   int* qidx;
   int* qold;
@@ -256,7 +257,7 @@ struct BoundedWalker {
       idx[i] = qidx[v[i] ? e : 0];
       old[i] = v[i] ? qold[e] : -1;
     }
-    reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, false,
+    reassign<CPL, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v, false,
                          old, slice, run);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -303,7 +304,7 @@ struct BoundedWalker {
       tile_pixels(g, idx, v, lvec);
 #pragma unroll
       for (int i = 0; i < 4; ++i) old[i] = v[i] ? old[i] : -1;
-      reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, lvec,
+      reassign<CPL, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v, lvec,
                            old, slice, run);
       return;
     }
@@ -406,7 +407,7 @@ struct BoundedWalker {
         idx[i] = v[i] ? q[e] : 0;
         old[i] = v[i] ? q[kQueueCap + e] : -1;
       }
-      reassign<CPL, false>(a, cx, cy, cz, cn, fl, fscale, lane, idx, v, false,
+      reassign<CPL, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v, false,
                            old, slice, run);
     }
     qlen = 0;
@@ -437,8 +438,8 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a, int s) {
   int* qold = qidx + kQueueCap;
   int2* list = reinterpret_cast<int2*>(
       reinterpret_cast<int*>(fl + kMaxTileK) + nwaves * 2 * kQueueCap);
-  const float maxabs = __uint_as_float(*a.maxabs);
-  const float fscale = scale_of(*a.maxabs);
+  const float maxabs = qmax_of(a.maxabs);
+  const QScale qs = qscale_of(a.maxabs);
   const float margin = 1e-5f * (1.f + 4.f * maxabs);
 
   // this launch's centroids c_s (and F_s in fl): the initial ones for s = 1,
@@ -452,7 +453,7 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a, int s) {
     long long* tot = reinterpret_cast<long long*>(list);
     double* d2 = reinterpret_cast<double*>(tot + 4 * kMaxTileK);
     float* cs = reinterpret_cast<float*>(d2 + kMaxTileK);
-    if (tiles_prologue(a, s, fscale, tot, d2, cs, fl)) return;
+    if (tiles_prologue(a, s, qs, tot, d2, cs, fl)) return;
     cent = cs;
   }
   for (int i = threadIdx.x; i < nwaves * V; i += blockDim.x) slices[i] = 0;
@@ -477,11 +478,11 @@ __global__ __launch_bounds__(1024) void tiles_step(TileArgs a, int s) {
     const int64_t W = (int64_t)gridDim.x * nwaves;
     walk_tiles<CPL, MODE, FULL>(
         a, TileRange{a.nseg * gw / W, a.nseg * (gw + 1) / W}, cx, cy, cz, cn,
-        fl, fscale, margin, slice, qidx, qold, lane);
+        fl, qs, margin, slice, qidx, qold, lane);
   } else {
     __shared__ unsigned unit_next;
     BoundedWalker<CPL, MODE> w{a,     cx,    cy,   cz,   cn,
-                               fl,    fscale, margin, slice, qidx,
+                               fl,    qs,    margin, slice, qidx,
                                qold,  lane,  (lane >> 2) * a.width + 4 * (lane & 3),
                                IRun{0, 0, 0, 0, -1, 0}, 0};
     const int64_t G = gridDim.x;
@@ -529,38 +530,49 @@ __global__ void tiles_init(LloydState* st, float* cbuf, const float* cent,
     st->threshold = threshold;
     st->shift = 0.f;
     st->next = 0u;
-    *maxabs = 0u;
   }
+  if (threadIdx.x < 3) maxabs[threadIdx.x] = 0u;
 }
 
-// Largest |v| over the three planes (as float bits; non-negative floats
-// order like their bit patterns).
-__global__ __launch_bounds__(256) void tiles_absmax(const float* pts,
-                                                    int64_t count, int vec,
-                                                    unsigned* out) {
+// Largest |v| of each of the three planes of n floats (plane c at pts +
+// c n) as float bits into out[c] (non-negative floats order like their bit
+// patterns).  blockIdx.y is the plane; a plane's 16-byte-aligned middle is
+// read as float4, four loads in flight per thread.
+__global__ __launch_bounds__(256) void planes_absmax(const float* pts,
+                                                     int64_t n,
+                                                     unsigned* out) {
+  const float* p = pts + (int64_t)blockIdx.y * n;
+  // [0, head): elements before the first 16-byte boundary (all of them
+  // when p is not 4-byte aligned), then whole float4s, then the tail
+  // [tail0, n)
+  const bool al = ((uintptr_t)p & 3u) == 0;
+  const int64_t head =
+      al ? std::min<int64_t>(
+               n, (int64_t)(((16u - ((uintptr_t)p & 15u)) & 15u) / 4u))
+         : n;
+  const float4* q4 = reinterpret_cast<const float4*>(p + head);
+  const int64_t chunks = (n - head) / 4;
+  const int64_t tail0 = head + 4 * chunks;
   float m = 0.f;
-  const int64_t chunks = vec ? count / 4 : 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  // four independent loads in flight per thread
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t c = t;
   for (; c + 3 * stride < chunks; c += 4 * stride) {
     float4 q[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      q[u] = reinterpret_cast<const float4*>(pts)[c + u * stride];
+    for (int u = 0; u < 4; ++u) q[u] = q4[c + u * stride];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
       m = fmaxf(m, fmaxf(fmaxf(fabsf(q[u].x), fabsf(q[u].y)),
                          fmaxf(fabsf(q[u].z), fabsf(q[u].w))));
   }
   for (; c < chunks; c += stride) {
-    const float4 q = reinterpret_cast<const float4*>(pts)[c];
+    const float4 q = q4[c];
     m = fmaxf(m, fmaxf(fmaxf(fabsf(q.x), fabsf(q.y)),
                        fmaxf(fabsf(q.z), fabsf(q.w))));
   }
-  for (int64_t i = 4 * chunks + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-       i < count; i += stride)
-    m = fmaxf(m, fabsf(pts[i]));
+  for (int64_t i = t; i < head; i += stride) m = fmaxf(m, fabsf(p[i]));
+  for (int64_t i = tail0 + t; i < n; i += stride) m = fmaxf(m, fabsf(p[i]));
   // one atomic per block (many blocks updating one address serialise)
   __shared__ float part[256 / kWave];
   m = wave_allreduce(m, OpMax());
@@ -568,18 +580,18 @@ __global__ __launch_bounds__(256) void tiles_absmax(const float* pts,
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)blockDim.x / kWave; ++w) m = fmaxf(m, part[w]);
-    atomicMax(out, __float_as_uint(m));
+    atomicMax(out + blockIdx.y, __float_as_uint(m));
   }
 }
 
-// tiles_absmax plus the interleaved copy of the pixels ({x, y, z, 0}) that
+// planes_absmax plus the interleaved copy of the pixels ({x, y, z, 0}) that
 // the queued pixels are read from.
 __global__ __launch_bounds__(256) void tiles_absmax_pack(const float* pts,
                                                          int64_t n, F4* aos,
                                                          unsigned* out) {
   // one pixel per lane and step: each plane load reads 256 contiguous bytes
   // and each store writes 1 KiB, four steps in flight
-  float m = 0.f;
+  float mx = 0.f, my = 0.f, mz = 0.f;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; i + 3 * stride < n; i += 4 * stride) {
@@ -594,21 +606,33 @@ __global__ __launch_bounds__(256) void tiles_absmax_pack(const float* pts,
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       aos[i + u * stride] = F4{x[u], y[u], z[u], 0.f};
-      m = fmaxf(m, fmaxf(fabsf(x[u]), fmaxf(fabsf(y[u]), fabsf(z[u]))));
+      mx = fmaxf(mx, fabsf(x[u]));
+      my = fmaxf(my, fabsf(y[u]));
+      mz = fmaxf(mz, fabsf(z[u]));
     }
   }
   for (; i < n; i += stride) {
     const float x = pts[i], y = pts[n + i], z = pts[2 * n + i];
     aos[i] = F4{x, y, z, 0.f};
-    m = fmaxf(m, fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z))));
+    mx = fmaxf(mx, fabsf(x));
+    my = fmaxf(my, fabsf(y));
+    mz = fmaxf(mz, fabsf(z));
   }
-  __shared__ float part[256 / kWave];
-  m = wave_allreduce(m, OpMax());
-  if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = m;
+  __shared__ float part[3][256 / kWave];
+  mx = wave_allreduce(mx, OpMax());
+  my = wave_allreduce(my, OpMax());
+  mz = wave_allreduce(mz, OpMax());
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    part[0][threadIdx.x / kWave] = mx;
+    part[1][threadIdx.x / kWave] = my;
+    part[2][threadIdx.x / kWave] = mz;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)blockDim.x / kWave; ++w) m = fmaxf(m, part[w]);
-    atomicMax(out, __float_as_uint(m));
+  if (threadIdx.x < 3) {
+    float m = part[threadIdx.x][0];
+    for (int w = 1; w < (int)blockDim.x / kWave; ++w)
+      m = fmaxf(m, part[threadIdx.x][w]);
+    atomicMax(out + threadIdx.x, __float_as_uint(m));
   }
 }
 
@@ -671,7 +695,7 @@ static TileLayout tiles_layout(void* temp, int64_t n, int k) {
       take((size_t)(2 + 3 * kAccCopies) * 4 * k * 8));
   L.partials = L.totals ? L.totals + 2 * 4 * (size_t)k : nullptr;
   L.fcum = reinterpret_cast<float*>(take(2 * (size_t)k * sizeof(float)));
-  L.maxabs = reinterpret_cast<unsigned*>(take(sizeof(unsigned)));
+  L.maxabs = reinterpret_cast<unsigned*>(take(3 * sizeof(unsigned)));
   L.pstate = reinterpret_cast<uint32_t*>(
       take(((size_t)n + kStatePad) * sizeof(uint32_t)));
   L.aos = reinterpret_cast<F4*>(take((size_t)n * sizeof(F4)));
@@ -696,12 +720,13 @@ static int64_t spread_multiplier(int64_t n) {
   return m;
 }
 
-int launch_absmax(const float* pts, int64_t count, unsigned* out,
+int launch_absmax(const float* pts, int64_t n, unsigned* out,
                   hipStream_t stream) {
-  const int vec = ((uintptr_t)pts & 15u) == 0;
-  hipLaunchKernelGGL(tiles_absmax, dim3(8 * device_cu_count()), dim3(256), 0,
-                     stream, pts, count, vec, out);
-  SPLIT_HIP_CHECK_LAUNCH("absmax");
+  const int64_t per_plane = std::max<int64_t>(
+      1, std::min<int64_t>((n / 4 + 1023) / 1024, 3 * device_cu_count()));
+  hipLaunchKernelGGL(planes_absmax, dim3((unsigned)per_plane, 3), dim3(256), 0,
+                     stream, pts, n, out);
+  SPLIT_HIP_CHECK_LAUNCH("planes_absmax");
   return SPLIT_HIP_OK;
 }
 

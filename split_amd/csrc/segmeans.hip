@@ -2,14 +2,14 @@
 // kernels hold (k = number of connected components, 10^4..10^7), and
 // propagate_centroids.
 //
-// Sums are exact integers: each channel is quantised once as
-// q = trunc(v * 2^(22 - e)) (e from the largest |v| of the points, as in the
+// Sums are exact integers: each channel is quantised once with its own
+// scale, q = rint(v * s_c) (s_c from the channel's largest |v|, as in the
 // tiled Lloyd path), a wave of 256 consecutive pixels reduces its pixels one
 // distinct label at a time (labels of connected components come in runs, so
 // a wave holds few), and one lane adds the run's sums and count to int64 /
 // uint32 global totals with atomics.  Integer atomics make the result
 // independent of order; the quantisation moves a mean by at most
-// max|v| * 2^-22.  A final pass divides (double) and writes the centroids;
+// max|v_c| * 2^-23 in channel c.  A final pass divides (double) and writes the centroids;
 // labels with no pixel get the zero centroid.
 #include <algorithm>
 
@@ -28,7 +28,7 @@ __global__ void sm_zero(long long* sums, unsigned* cnt, int64_t k,
     sums[i] = 0;
     if (i < k) cnt[i] = 0u;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *maxabs = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < 3) maxabs[threadIdx.x] = 0u;
 }
 
 // Loads pixel indices base + 4*lane + i (vec) or base + lane + 64*i.
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kSmThreads) void sm_accumulate(
     int64_t n, int64_t k, int vec, const unsigned* maxabs, long long* sums,
     unsigned* cnt) {
   const int lane = threadIdx.x & (kWave - 1);
-  const float fscale = scale_of(*maxabs);
+  const QScale qs = qscale_of(maxabs);
   const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
   for (int64_t s = w0; s * 256 < n; s += nw) {
@@ -77,9 +77,9 @@ __global__ __launch_bounds__(kSmThreads) void sm_accumulate(
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       rem |= (lab[i] >= 0 && (int64_t)lab[i] < k ? 1u : 0u) << i;
-      qx[i] = (int)(x[i] * fscale);
-      qy[i] = (int)(y[i] * fscale);
-      qz[i] = (int)(z[i] * fscale);
+      qx[i] = quantise(x[i], qs.x);
+      qy[i] = quantise(y[i], qs.y);
+      qz[i] = quantise(z[i], qs.z);
     }
     for (;;) {  // one distinct label per round
       const uint64_t act = ballot(rem != 0);
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kSmThreads) void sm_accumulate(
         c += m ? 1 : 0;
         rem &= m ? ~(1u << i) : ~0u;
       }
-      // |per-lane sum| <= 4 * 2^22, so a wave sum fits int32
+      // |per-lane sum| <= 4 (2^23 - 1), so a wave sum fits int32
       const long long tx = wave_sum_i32(sx), ty = wave_sum_i32(sy),
                       tz = wave_sum_i32(sz);
       const unsigned tc = (unsigned)wave_sum_i32(c);
@@ -120,14 +120,14 @@ __global__ __launch_bounds__(kSmThreads) void sm_accumulate(
 __global__ void sm_finalize(const long long* sums, const unsigned* cnt,
                             int64_t k, const unsigned* maxabs,
                             float* centroids) {
-  const double inv = 1.0 / (double)scale_of(*maxabs);
+  const QScale qs = qscale_of(maxabs);
+  const float sc[3] = {qs.x, qs.y, qs.z};
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < k;
        j += (int64_t)gridDim.x * blockDim.x) {
     const unsigned c = cnt[j];
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch)
-      centroids[ch * k + j] =
-          c ? (float)((double)sums[ch * k + j] * inv / (double)c) : 0.f;
+      centroids[ch * k + j] = dequantise_mean(sums[ch * k + j], c, sc[ch]);
   }
 }
 
@@ -160,7 +160,7 @@ SmLayout sm_layout(void* temp, int64_t k) {
     o += align_up(bytes, 256);
     return p;
   };
-  L.maxabs = reinterpret_cast<unsigned*>(take(sizeof(unsigned)));
+  L.maxabs = reinterpret_cast<unsigned*>(take(3 * sizeof(unsigned)));
   L.sums = reinterpret_cast<long long*>(take(3 * (size_t)k * 8));
   L.cnt = reinterpret_cast<unsigned*>(take((size_t)k * 4));
   L.bytes = o;
@@ -187,7 +187,7 @@ int segment_means(const int32_t* labels, const float* points, int64_t n,
                 temp_bytes, (long long)k, L.bytes);
   hipLaunchKernelGGL(sm_zero, dim3(grid_of(3 * k, 8)), dim3(256), 0, stream,
                      L.sums, L.cnt, k, L.maxabs);
-  SPLIT_HIP_TRY(launch_absmax(points, 3 * n, L.maxabs, stream));
+  SPLIT_HIP_TRY(launch_absmax(points, n, L.maxabs, stream));
   const int vec = (((uintptr_t)points & 15u) == 0) &&
                   (((uintptr_t)labels & 15u) == 0) && (n % 4 == 0);
   hipLaunchKernelGGL(sm_accumulate, dim3(grid_of(n / 4 + 1, 16)),

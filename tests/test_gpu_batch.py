@@ -158,3 +158,35 @@ def test_batch_errors(torch_cuda, batch_images):
     with pytest.raises(sa.SplitError) as e:
         sa.segment_batch([im], max_iter=5, temp=small)
     assert e.value.status == 2  # TEMP
+
+
+def test_batch_graph_key_cc_tiles(torch_cuda):
+    """Two one-image batches with the same k, Lloyd tile count (9) and padded
+    pixel count (2304) but different counts of 32x32 CC tiles (48x48: 4,
+    144x16: 5) share one temp: the second call must capture its own graph,
+    and its segments equal the CPU union-find."""
+    torch = torch_cuda
+    imgs = []
+    for i, (w, h) in enumerate([(48, 48), (144, 16)]):
+        spec = W.ImageSpec(w, h, seed=0x6EA7_0000 + i, sites=12, shading=True,
+                           noise_sigma=0.01, base_cell=max(w, h) / 4.0)
+        P = torch.from_numpy(W.make_image(spec)).cuda()
+        C0 = torch.empty((3, 8), dtype=torch.float32, device="cuda")
+        sa.uniform_random_initialize(P, C0, 0x6EA7 + i)
+        imgs.append(dict(points=P, width=w, height=h, init=C0))
+    sizes = [sa.segment_batch_workspace(_fresh(torch, [im])) for im in imgs]
+    assert sizes[0] == sizes[1]
+    temp = torch.empty(max(sizes), dtype=torch.uint8, device="cuda")
+    for rep in range(2):
+        for im in imgs:
+            say(f"batch graph key: {im['width']}x{im['height']} pass {rep}")
+            got = _fresh(torch, [im])
+            res = sa.segment_batch(got, max_iter=10, threshold=-1.0, connectivity=8,
+                                   compress=True, use_graph=True, temp=temp)
+            torch.cuda.synchronize()
+            g = got[0]
+            lab = g["labels"].cpu().numpy().reshape(g["height"], g["width"])
+            rep_, ncomp = oracle.cc_uf(lab, 8)
+            want, cnt = oracle.compress(rep_)
+            assert res[0].num_segments == cnt == ncomp
+            assert np.array_equal(g["segments"].cpu().numpy(), want)
