@@ -95,13 +95,30 @@ size_t split_hip_calculate_centroids_workspace(int64_t n, int32_t k);
  * centroids.cu:65-67).  Labels outside [0, k) contribute nothing.
  * Counts are exact integers.  k <= SPLIT_HIP_MAX_K: float sums reduced in a
  * fixed order.  Larger k (segment means, k = number of segments, the
- * caller's use at samples/src/main.cu:255-307): sums of channels quantised
- * to 2^-(22-e) (max|v| <= 2^e) accumulated exactly in int64, so the result
- * does not depend on order; the temp needed is about 28 bytes per label. */
+ * caller's use at samples/src/main.cu:255-307): each channel c quantised
+ * with its own scale, q = rint(v * 2^(23-e_c)) (max|v_c| < 2^e_c), and
+ * accumulated exactly in int64, so the result does not depend on order and
+ * a mean moves by at most max|v_c| * 2^-23; the temp needed is about 28
+ * bytes per label. */
 int split_hip_calculate_centroids(const int32_t* labels, const float* points,
                                   int64_t n, float* centroids, int32_t k,
                                   void* temp, size_t temp_bytes,
                                   split_hip_stream_t stream);
+
+/* No counterpart in the reference.  split_hip_calculate_centroids for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same kernels and
+ * results, with their scratch in a static device array of
+ * 16 MiB + 256 bytes that each GPU holds until the process ends, instead of
+ * a temp block.  The caller passes only inputs and outputs.  Any
+ * k <= SPLIT_HIP_MAX_K (bit-identical to split_hip_calculate_centroids with
+ * its workspace); larger k up to about 599,000 labels; beyond that it
+ * returns SPLIT_HIP_ERR_UNSUPPORTED.  Calls that overlap on one GPU share the
+ * array and race: serialise them (one stream, one host thread). */
+int split_hip_calculate_centroids_noalloc(const int32_t* labels,
+                                          const float* points, int64_t n,
+                                          float* centroids, int32_t k,
+                                          split_hip_stream_t stream);
 
 /* Replaces split::device::kmeans::propagate_centroids
  * (split/include/split/device/kmeans/propagation.cuh:24-27,

@@ -101,6 +101,8 @@ CABI = {
     "split_hip_calculate_centroids_workspace": (_c_size_t, [_c_i64, _c_i32]),
     "split_hip_calculate_centroids": (ctypes.c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p,
                                                      _c_i32, _c_void_p, _c_size_t, _c_void_p]),
+    "split_hip_calculate_centroids_noalloc": (ctypes.c_int, [_c_void_p, _c_void_p, _c_i64,
+                                                             _c_void_p, _c_i32, _c_void_p]),
     "split_hip_propagate_centroids": (ctypes.c_int, [_c_void_p, _c_i64, _c_void_p, _c_i64,
                                                      _c_void_p, _c_void_p]),
     "split_hip_cluster_workspace": (_c_size_t, [_c_i64, _c_i32, _c_i32]),
@@ -283,6 +285,20 @@ def calculate_centroids(labels, points, centroids, temp=None) -> None:
                                            _stream()), "calculate_centroids")
 
 
+def calculate_centroids_noalloc(labels, points, centroids) -> None:
+    """calculate_centroids with no temp and no device allocation: the kernels
+    use the library's static per-GPU scratch (calls that overlap on one GPU
+    race)."""
+    torch = _torch()
+    p, n = _points(points)
+    c, k = _cent(centroids)
+    lb = _dev(labels, torch.int32, "labels")
+    if labels.numel() != n:
+        raise ValueError("labels must hold N entries")
+    _check(lib().split_hip_calculate_centroids_noalloc(lb, p, n, c, k, _stream()),
+           "calculate_centroids_noalloc")
+
+
 def propagate_centroids(labels, centroids, points) -> None:
     """split::device::kmeans::propagate_centroids (propagation.cu:9-24):
     points[:, i] = centroids[:, labels[i]] (labels outside [0, K) untouched)."""
@@ -333,6 +349,7 @@ def uniform_random_initialize(points, centroids, seed: int) -> None:
 
 
 kmeans = SimpleNamespace(label_points=label_points, calculate_centroids=calculate_centroids,
+                         calculate_centroids_noalloc=calculate_centroids_noalloc,
                          cluster=cluster, uniform_random_initialize=uniform_random_initialize,
                          propagate_centroids=propagate_centroids)
 

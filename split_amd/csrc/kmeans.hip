@@ -630,6 +630,80 @@ int cluster_async(const float* pts, int64_t n, int width, int height,
   return SPLIT_HIP_OK;
 }
 
+int calculate_centroids_impl(const int32_t* labels, const float* points,
+                             int64_t n, float* centroids, int32_t k,
+                             void* temp, size_t temp_bytes,
+                             hipStream_t stream) {
+  if (k > SPLIT_HIP_MAX_K) {  // segment means: k beyond the register kernels
+    if (!points || !labels || !centroids || !temp || n < 1)
+      return fail(SPLIT_HIP_ERR_ARGUMENT,
+                  "points, labels, centroids or temp is null, or n < 1");
+    return segment_means(labels, points, n, centroids, k, temp, temp_bytes,
+                         stream);
+  }
+  SPLIT_HIP_TRY(check_points(points, n, k));
+  if (!labels || !centroids || !temp)
+    return fail(SPLIT_HIP_ERR_ARGUMENT, "labels, centroids or temp is null");
+  if (temp_bytes < kStateBytes + partial_bytes_min(k))
+    return fail(SPLIT_HIP_ERR_TEMP,
+                "temp holds %zu bytes, calculate_centroids needs >= %zu",
+                temp_bytes, kStateBytes + partial_bytes_min(k));
+  char* t = static_cast<char*>(temp);
+  StepArgs a{};
+  a.pts = points;
+  set_geometry(a, n, 0, 0);
+  a.k = k;
+  a.labels_in = labels;
+  a.cent_out = centroids;
+  a.arrive = reinterpret_cast<unsigned*>(t);
+  a.partial = reinterpret_cast<float*>(t + kStateBytes);
+  hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, stream, a.arrive, 1);
+  SPLIT_HIP_CHECK_LAUNCH("zero_u32");
+  const StepCfg c = step_cfg(a, true, temp_bytes - kStateBytes);
+  return launch_step<false, true>(a, c, stream);
+}
+
+// ------------------------------------------- calculate_centroids_noalloc ---
+//
+// Scratch of the allocation-free calculate_centroids: a file-scope device
+// array that each GPU holds for the life of the process.  Its size is the
+// workspace calculate_centroids asks for at k = SPLIT_HIP_MAX_K (arrival
+// counter + partial columns for the full grid, 16 MiB + 256 B), so every
+// k <= SPLIT_HIP_MAX_K runs the same grid as the allocating entry point;
+// larger k (segment means, ~28 B per label) fits up to kNoAllocMaxSegments.
+// Every call on a GPU uses that GPU's one array: the kernels clear the
+// counter and sums they need at the start of each call, and calls that
+// overlap on one GPU (two streams, two host threads) race.
+constexpr size_t kNoAllocScratchBytes =
+    kStateBytes + (size_t)4 * SPLIT_HIP_MAX_K * kMaxGrid * sizeof(float);
+constexpr int64_t kNoAllocMaxSegments =
+    (int64_t)(kNoAllocScratchBytes - 1024) / 28;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kNoAllocMaxDevices = 64;
+
+// The device address of this GPU's scratch, looked up once per device (with
+// that device current) and cached.  The array's name in host code is only a
+// stand-in for the symbol: kernels and copies get the address from here.
+static int noalloc_scratch(void** out) {
+  static std::mutex mu;
+  static void* ptrs[kNoAllocMaxDevices] = {};
+  int dev = 0;
+  SPLIT_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices)
+    return fail(SPLIT_HIP_ERR_UNSUPPORTED, "device %d beyond %d", dev,
+                kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    SPLIT_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p)
+      return fail(SPLIT_HIP_ERR_HIP, "hipGetSymbolAddress gave no address");
+    ptrs[dev] = p;
+  }
+  *out = ptrs[dev];
+  return SPLIT_HIP_OK;
+}
+
 }  // namespace splithip
 
 using namespace splithip;
@@ -682,33 +756,24 @@ int split_hip_calculate_centroids(const int32_t* labels, const float* points,
                                   int64_t n, float* centroids, int32_t k,
                                   void* temp, size_t temp_bytes,
                                   split_hip_stream_t stream) {
-  if (k > SPLIT_HIP_MAX_K) {  // segment means: k beyond the register kernels
-    if (!points || !labels || !centroids || !temp || n < 1)
-      return fail(SPLIT_HIP_ERR_ARGUMENT,
-                  "points, labels, centroids or temp is null, or n < 1");
-    return segment_means(labels, points, n, centroids, k, temp, temp_bytes,
-                         stream);
-  }
-  SPLIT_HIP_TRY(check_points(points, n, k));
-  if (!labels || !centroids || !temp)
-    return fail(SPLIT_HIP_ERR_ARGUMENT, "labels, centroids or temp is null");
-  if (temp_bytes < kStateBytes + partial_bytes_min(k))
-    return fail(SPLIT_HIP_ERR_TEMP,
-                "temp holds %zu bytes, calculate_centroids needs >= %zu",
-                temp_bytes, kStateBytes + partial_bytes_min(k));
-  char* t = static_cast<char*>(temp);
-  StepArgs a{};
-  a.pts = points;
-  set_geometry(a, n, 0, 0);
-  a.k = k;
-  a.labels_in = labels;
-  a.cent_out = centroids;
-  a.arrive = reinterpret_cast<unsigned*>(t);
-  a.partial = reinterpret_cast<float*>(t + kStateBytes);
-  hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, stream, a.arrive, 1);
-  SPLIT_HIP_CHECK_LAUNCH("zero_u32");
-  const StepCfg c = step_cfg(a, true, temp_bytes - kStateBytes);
-  return launch_step<false, true>(a, c, stream);
+  return calculate_centroids_impl(labels, points, n, centroids, k, temp,
+                                  temp_bytes, stream);
+}
+
+int split_hip_calculate_centroids_noalloc(const int32_t* labels,
+                                          const float* points, int64_t n,
+                                          float* centroids, int32_t k,
+                                          split_hip_stream_t stream) {
+  if (k > SPLIT_HIP_MAX_K && segment_means_workspace(k) > kNoAllocScratchBytes)
+    return fail(SPLIT_HIP_ERR_UNSUPPORTED,
+                "calculate_centroids_noalloc: k = %d needs %zu bytes of "
+                "scratch, the static scratch holds %zu (k <= %lld)",
+                k, segment_means_workspace(k), kNoAllocScratchBytes,
+                (long long)kNoAllocMaxSegments);
+  void* scratch_ptr = nullptr;
+  SPLIT_HIP_TRY(noalloc_scratch(&scratch_ptr));
+  return calculate_centroids_impl(labels, points, n, centroids, k, scratch_ptr,
+                                  kNoAllocScratchBytes, stream);
 }
 
 size_t split_hip_cluster_workspace(int64_t n, int32_t k, int32_t) {
