@@ -35,8 +35,9 @@ cluster(const_view2d<real> di_points,
         real i_threshold = 1.f);
 
 /// Temp bytes for cluster: at least the reference's formula
-/// (cluster.cu:12-25), so that the same block also serves
-/// connected_components and compress_labels afterwards.
+/// (cluster.cu:12-25), and enough that the same block also serves
+/// connected_components, compress_labels and calculate_centroids with any
+/// K <= N afterwards (the sample's reuse, main.cu:240-307).
 SPLIT_API std::size_t
 cluster_workspace(const int i_npoints, const int i_nclusters, const int i_dim);
 

@@ -8,6 +8,11 @@ Outputs (git-ignored, shipped with the tree):
     split_amd/_lib/libsplit_hip.so   HIP kernels + C ABI (include/split_hip.h)
                                      + C++ drop-in wrappers (include/split/...)
     split_amd/_lib/libsplit_gen.so   seeded image generators (host C)
+    split_amd/_lib/split_dropin_sequence
+                                     the reference sample's call sequence
+                                     against the drop-in headers
+                                     (tests/dropin/main_sequence.cpp), run
+                                     by the GPU tests
 """
 from __future__ import annotations
 
@@ -93,8 +98,19 @@ def build_gen() -> Path:
     return out
 
 
+def build_dropin() -> Path:
+    """The C++ caller of the drop-in headers, linked to libsplit_hip.so."""
+    out = LIB / "split_dropin_sequence"
+    tmp = LIB / "split_dropin_sequence.tmp"
+    _run([hipcc(), "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}",
+          str(ROOT / "tests" / "dropin" / "main_sequence.cpp"), f"-L{LIB}",
+          "-lsplit_hip", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(jobs: int = 4) -> list[Path]:
-    return [build_gen(), build_hip(jobs)]
+    return [build_gen(), build_hip(jobs), build_dropin()]
 
 
 def main() -> None:

@@ -172,6 +172,7 @@ __device__ __forceinline__ Geometry image_geometry(const BatchImg& m) {
   g.n = (int64_t)m.W * m.H;
   g.width = m.W;
   g.height = m.H;
+  g.full_rows = m.H;
   g.tiles_x = m.tiles_x;
   g.nseg = (int64_t)m.tiles_x * ((m.H + kTileSide - 1) / kTileSide);
   return g;

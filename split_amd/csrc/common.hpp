@@ -202,7 +202,9 @@ int segment_means(const int32_t* labels, const float* points, int64_t n,
                   hipStream_t stream);
 size_t tiles_workspace(int64_t n, int k);
 bool tiles_supported(int64_t n, int width, int k);
-int cluster_tiles_async(const float* pts, int width, int height,
+// Bounded Lloyd over the n pixels as rows of `width` (the last row may be
+// partial).
+int cluster_tiles_async(const float* pts, int64_t n, int width,
                         float* centroids, int k, int32_t* labels, void* temp,
                         size_t temp_bytes, int max_iter, float threshold,
                         hipStream_t stream, Mailbox** mbox_out);

@@ -498,10 +498,11 @@ static void set_geometry(StepArgs& a, int64_t n, int width, int height) {
   if (width > 0) {
     a.width = width;
     a.height = height;
+    a.full_rows = height;
     a.tiles_x = (width + kTileSide - 1) / kTileSide;
     a.nseg = (int64_t)a.tiles_x * ((height + kTileSide - 1) / kTileSide);
   } else {
-    a.width = a.height = a.tiles_x = 0;
+    a.width = a.height = a.tiles_x = a.full_rows = 0;
     a.nseg = (n + kSegPx - 1) / kSegPx;
   }
 }
@@ -553,8 +554,16 @@ static int check_points(const float* pts, int64_t n, int k) {
   return SPLIT_HIP_OK;
 }
 
+// Width of the rows a point set without image geometry is walked in by the
+// bounded tiled path: a 16x16 tile is then 256 consecutive points.
+constexpr int kPointsWidth = 16;
+
 // Internal entry used by the pipeline: Lloyd loop without the final host
 // synchronisation (the mailbox holds the result once the stream drains).
+// width == 0: a point set without image geometry.  The bounded tiled path
+// (k <= 256) takes it as rows of kPointsWidth points; the 1-D runs path
+// (float sums) serves larger k and temp blocks too small for the tiled
+// path's per-pixel state.
 int cluster_async(const float* pts, int64_t n, int width, int height,
                   float* centroids, int k, int32_t* labels, void* temp,
                   size_t temp_bytes, int max_iter, float threshold,
@@ -562,9 +571,9 @@ int cluster_async(const float* pts, int64_t n, int width, int height,
   SPLIT_HIP_TRY(check_points(pts, n, k));
   if (!centroids) return fail(SPLIT_HIP_ERR_ARGUMENT, "centroids is null");
   if (!temp) return fail(SPLIT_HIP_ERR_ARGUMENT, "temp is null");
-  if (tiles_supported(n, width, k) &&
-      temp_bytes >= tiles_workspace(n, k))
-    return cluster_tiles_async(pts, width, height, centroids, k, labels, temp,
+  const int tw = width > 0 ? width : kPointsWidth;
+  if (tiles_supported(n, tw, k) && temp_bytes >= tiles_workspace(n, k))
+    return cluster_tiles_async(pts, n, tw, centroids, k, labels, temp,
                                temp_bytes, max_iter, threshold, stream,
                                mbox_out);
   const size_t head = kStateBytes + align_up(6 * (size_t)k * sizeof(float), 256);

@@ -298,7 +298,7 @@ __device__ __forceinline__ void walk_tiles(
       const int row = p.ty * kTileSide + (lane >> 2);
       const int col = p.tx * kTileSide + 4 * (lane & 3);
       const int64_t o = (int64_t)row * a.width + col;
-      if ((p.ty + 1) * kTileSide <= a.height &&
+      if ((p.ty + 1) * kTileSide <= a.full_rows &&
           (p.tx + 1) * kTileSide <= a.width) {
         // every pixel exists: no per-lane guards
         lvec = MODE == kTileVec;
@@ -312,12 +312,12 @@ __device__ __forceinline__ void walk_tiles(
           nfail += __builtin_popcountll(fb[i]);
         }
       } else {
-        const bool rowok = row < a.height;
-        lvec = MODE == kTileVec && rowok && col + 3 < a.width;
+        const int rp = row_pixels(a, row);
+        lvec = MODE == kTileVec && col + 3 < rp;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           idx[i] = o + i;
-          v[i] = rowok && col + i < a.width;
+          v[i] = col + i < rp;
           old[i] = v[i] ? (int)(ps[i] & 0xFFu) : -1;
           fail[i] = v[i] &&
                     !(state_budget(ps[i]) - fl[v[i] ? old[i] : 0] > margin);

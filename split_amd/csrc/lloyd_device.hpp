@@ -16,8 +16,11 @@ struct Geometry {
   int64_t n;
   int64_t nseg;      // segments: ceil(n / kSegPx) or tile count
   int width;         // > 0: 16x16-tile traversal of width x height
-  int height;
+  int height;        // rows: ceil(n / width)
   int tiles_x;       // ceil(width / 16)
+  // rows that hold `width` pixels: height for an image; n / width for a
+  // point set walked as a width-16 image, whose last row may be partial
+  int full_rows;
   // optional interleaved copy of the pixels ({x, y, z, 0} per pixel) for
   // scattered reads: one 16-byte load touches one cache line instead of
   // three (one per plane)
@@ -25,6 +28,14 @@ struct Geometry {
 };
 
 // ------------------------------------------------------------- loading ---
+
+// Pixels in row `row` of a tiled traversal: width, fewer in a partial last
+// row (n not a multiple of width), none at or below height.
+__device__ __forceinline__ int row_pixels(const Geometry& a, int row) {
+  if (row < a.full_rows) return a.width;
+  const int64_t left = a.n - (int64_t)row * a.width;
+  return left < (int64_t)a.width ? (int)(left > 0 ? left : 0) : a.width;
+}
 
 // Traversal modes: 1-D runs of 256 pixels (dword or float4 loads) or 16x16
 // tiles (dword or float4 loads per lane row).
@@ -45,13 +56,13 @@ __device__ __forceinline__ void segment_index(const Geometry& a, int64_t s,
     const int row = ty * kTileSide + (lane >> 2);
     const int col = tx * kTileSide + (lane & 3) * 4;
     const int64_t o = (int64_t)row * a.width + col;
-    const bool rowok = row < a.height;
+    const int rp = row_pixels(a, row);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      v[i] = rowok && col + i < a.width;
+      v[i] = col + i < rp;
       idx[i] = o + i;
     }
-    lvec = MODE == kTileVec && rowok && col + 3 < a.width;
+    lvec = MODE == kTileVec && col + 3 < rp;
   } else {
     const int64_t base = s * kSegPx;
     lvec = MODE == kRunVec && base + kSegPx <= a.n;

@@ -215,19 +215,19 @@ struct BoundedWalker {
                                               int64_t (&idx)[4], bool (&v)[4],
                                               bool& lvec) const {
     const int col = g.col0 + 4 * (lane & 3);
-    const bool rowok = g.row0 + (lane >> 2) < a.height;
+    const int rp = row_pixels(a, g.row0 + (lane >> 2));
     const int o = g.off + lane_off;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      v[i] = rowok && col + i < a.width;
+      v[i] = col + i < rp;
       idx[i] = o + i;
     }
-    lvec = MODE == kTileVec && rowok && col + 3 < a.width;
+    lvec = MODE == kTileVec && col + 3 < rp;
   }
 
   // Tiles wholly inside the image (wave-uniform: the geometry is scalar).
   __device__ __forceinline__ bool inner(const TileGeo& g) const {
-    return g.row0 + kTileSide <= a.height && g.col0 + kTileSide <= a.width;
+    return g.row0 + kTileSide <= a.full_rows && g.col0 + kTileSide <= a.width;
   }
 
   // One 16-byte load per lane on every path, so that the wait for this
@@ -274,7 +274,6 @@ struct BoundedWalker {
   __device__ __forceinline__ void visit(const TileGeo& g,
                                         const uint32_t (&ps)[4]) {
     const int col = g.col0 + 4 * (lane & 3);
-    const bool rowok = g.row0 + (lane >> 2) < a.height;
     int old[4];
     bool fail[4];
     uint64_t fb[4];
@@ -288,9 +287,10 @@ struct BoundedWalker {
         nfail += __builtin_popcountll(fb[i]);
       }
     } else {
+      const int rp = row_pixels(a, g.row0 + (lane >> 2));
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const bool v = rowok && col + i < a.width;
+        const bool v = col + i < rp;
         old[i] = (int)(ps[i] & 0xFFu);
         fail[i] = v && !(state_budget(ps[i]) - fl[v ? old[i] : 0] > margin);
         fb[i] = ballot(fail[i]);
@@ -658,11 +658,12 @@ __global__ void tiles_finish(const LloydState* st, const float* cbuf,
 static int tiles_threads(int k) { return k <= 128 ? 1024 : 256; }
 // Step launches use at most this many blocks.
 static int tiles_max_blocks(int k) { return k <= 128 ? 512 : 1024; }
-// Units (pairs of adjacent tiles) of a width x (n / width) image.
+// Units (pairs of adjacent tiles) of n pixels in rows of width.
 static int64_t tile_units(int64_t n, int width) {
   if (width <= 0) return 0;
   const int64_t tiles_x = (width + kTileSide - 1) / kTileSide;
-  const int64_t tiles_y = (n / width + kTileSide - 1) / kTileSide;
+  const int64_t rows = (n + width - 1) / width;
+  const int64_t tiles_y = (rows + kTileSide - 1) / kTileSide;
   return (tiles_x + 1) / 2 * tiles_y;
 }
 
@@ -758,11 +759,13 @@ static void tiles_launch_mode(const TileArgs& a, int s, int grid,
   }
 }
 
-int cluster_tiles_async(const float* pts, int width, int height,
+int cluster_tiles_async(const float* pts, int64_t n, int width,
                         float* centroids, int k, int32_t* labels, void* temp,
                         size_t temp_bytes, int max_iter, float threshold,
                         hipStream_t stream, Mailbox** mbox_out) {
-  const int64_t n = (int64_t)width * height;
+  // rows of width pixels; the last one may be partial (a point set walked
+  // as a width-16 image)
+  const int height = (int)((n + width - 1) / width);
   const TileLayout L = tiles_layout(temp, n, k);
   if (temp_bytes < L.bytes)
     return fail(SPLIT_HIP_ERR_TEMP,
@@ -786,6 +789,7 @@ int cluster_tiles_async(const float* pts, int width, int height,
   a.n = n;
   a.width = width;
   a.height = height;
+  a.full_rows = (int)(n / width);
   a.tiles_x = (width + kTileSide - 1) / kTileSide;
   a.nseg = (int64_t)a.tiles_x * ((height + kTileSide - 1) / kTileSide);
   a.k = k;

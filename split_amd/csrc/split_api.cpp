@@ -113,13 +113,14 @@ void calculate_centroids(const_view1d<int> di_labels,
   rows3(do_centroids.num_rows, "calculate_centroids");
   const int k = (int)do_centroids.num_cols;
   const std::int64_t n = di_points.num_cols;
-  // A reference caller sized temp by the reference's formula (at least
-  // N*2*sizeof(int) + K*sizeof(real)); the grid adapts to that size.
+  // The contract of this entry point: do_temp holds
+  // calculate_centroids_workspace(N, K) bytes (a block sized by
+  // cluster_workspace(N, ...) holds it for every K <= N, which covers the
+  // segment means of main.cu:293-307 in the clustering temp).
   check(split_hip_calculate_centroids(
             di_labels.data(), di_points.values.data(), n,
             do_centroids.values.data(), k, do_temp.get(),
-            (std::size_t)n * 2 * sizeof(int) + (std::size_t)k * sizeof(real),
-            nullptr),
+            calculate_centroids_workspace((int)n, k), nullptr),
         "calculate_centroids");
   sync("calculate_centroids");
 }
@@ -147,9 +148,13 @@ std::size_t cluster_workspace(const int i_npoints, const int i_nclusters,
       (std::size_t)i_nclusters * i_dim * sizeof(real) +
           calculate_centroids_workspace(i_npoints, i_nclusters),
       label_points_workspace(i_npoints, i_nclusters));
+  // The reference's sample reuses this block for connected_components,
+  // compress_labels and calculate_centroids with K = number of segments
+  // (<= N) (main.cu:240-307): cover all of them.
   return std::max({ref,
                    split_hip_cluster_workspace(i_npoints, i_nclusters, i_dim),
-                   split_hip_compress_labels_workspace(i_npoints, i_npoints)});
+                   split_hip_compress_labels_workspace(i_npoints, i_npoints),
+                   calculate_centroids_workspace(i_npoints, i_npoints)});
 }
 
 // split/src/split/device/kmeans/cluster.cu:27-93

@@ -459,3 +459,67 @@ def test_segment_plain_512(torch_cuda, plain):
     assert r.iterations == 20
     assert r.num_segments == cnt == ncomp
     assert np.array_equal(host(S), want)
+
+
+@pytest.mark.parametrize("n,k,iters", [(262144 - 7, 8, 20), (4097, 5, 15), (17, 3, 6),
+                                       (100_003, 32, 25)])
+def test_cluster_points_without_width(torch_cuda, n, k, iters):
+    """cluster() of a point set with no image width: the bounded tiled path
+    walks it as rows of 16 points, the last row partial when n is not a
+    multiple of 16.  Centroids within 1e-5 of the FP64 trajectory; labels
+    equal the FP64 argmin against the previous centroids except near ties,
+    and equal a dense assignment against the (iters - 1)-run's centroids."""
+    torch = torch_cuda
+    say(f"cluster points n={n} k={k} x{iters} (no width)")
+    spec = W.ImageSpec(512, 512, seed=0x9A66 + n, sites=64, noise_sigma=0.02)
+    img = np.ascontiguousarray(W.make_image(spec)[:, :n])
+    init = seeded_centroids(img, k, 0x1A66 + k)
+    P = dev(torch, img)
+    runs = {}
+    for n_it in (iters - 1, iters):
+        C = dev(torch, init)
+        L = torch.empty(n, dtype=torch.int32, device="cuda")
+        it, _ = sa.cluster(P, C, L, n_it, -1.0)
+        assert it == n_it
+        runs[n_it] = (host(C), host(L))
+    c_prev, _ = runs[iters - 1]
+    c_last, l_last = runs[iters]
+    want_c, want_l = oracle.lloyd_f64(img, init, iters)
+    assert np.abs(c_last - want_c).max() <= CENT_TOL
+    cp, _ = oracle.lloyd_f64(img, init, iters - 1)
+    _, gap = oracle.label_f64(img, cp.astype(np.float32))
+    assert not ((l_last != want_l) & (gap >= TIE_REL)).any()
+    L2 = torch.empty(n, dtype=torch.int32, device="cuda")
+    sa.label_points(dev(torch, c_prev), P, L2)
+    assert np.array_equal(host(L2), l_last)
+
+
+@pytest.mark.parametrize("k", [8, 300])
+def test_cluster_runs_path(torch_cuda, plain, k):
+    """The 1-D runs path (float sums): k > 256, or a temp block too small for
+    the tiled path's per-pixel state.  Centroids within 1e-5 of the FP64
+    means of the returned labels; labels equal a dense assignment against
+    the previous iteration's centroids."""
+    torch = torch_cuda
+    w, img = plain
+    n = img.shape[1]
+    iters = 8
+    say(f"cluster runs path plain_512 k={k} x{iters}")
+    init = seeded_centroids(img, k, 0x2B + k)
+    P = dev(torch, img)
+    # k = 8: a temp that holds the runs path's partial columns but not the
+    # tiled path's 20 bytes per pixel
+    temp = torch.empty(1 << 20, dtype=torch.uint8, device="cuda") if k == 8 else None
+    runs = {}
+    for n_it in (iters - 1, iters):
+        C = dev(torch, init)
+        L = torch.empty(n, dtype=torch.int32, device="cuda")
+        it, _ = sa.cluster(P, C, L, n_it, -1.0, temp=temp)
+        assert it == n_it
+        runs[n_it] = (host(C), host(L))
+    L2 = torch.empty(n, dtype=torch.int32, device="cuda")
+    sa.label_points(dev(torch, runs[iters - 1][0]), P, L2)
+    assert np.array_equal(host(L2), runs[iters][1])
+    m, cnt = oracle.centroids_f64(runs[iters][1], img, k)
+    live = cnt > 0
+    assert np.abs(runs[iters][0][:, live] - m[:, live]).max() <= CENT_TOL
