@@ -40,7 +40,7 @@ def _fp64_means(pts, labels, k):
     return out, cnt
 
 
-@pytest.mark.parametrize("name,iters", [("shaded_4096", 100), ("stress_8192", 6)])
+@pytest.mark.parametrize("name,iters", [("shaded_4096", 100), ("stress_8192", 24)])
 def test_fullsize_bounded_lloyd(torch_cuda, name, iters):
     torch = torch_cuda
     w = W.WORKLOADS[name]
@@ -101,3 +101,67 @@ def test_fullsize_cc_stress_8192(torch_cuda):
     rep, ncomp = oracle.cc_uf(lab, 8)
     assert np.array_equal(out.cpu().numpy(), rep)
     assert sa.compress_labels(out) == ncomp
+
+
+def test_fullsize_batch_256(torch_cuda):
+    """The batched configuration at its size: all 256 images of
+    batch_workloads() (sides 128..2048, k in {4..64}, ~3e8 pixels) in one
+    segment_batch call (shift < 1e-4 or 100 iterations, 8-connectivity).
+    Per image, against the oracle and properties, not against the single-
+    image path: centroids within 1e-5 of the FP64 means of the image's own
+    labels; labels equal a dense assignment against the centroids after
+    I - 1 iterations (I = the image's iteration count; those centroids from
+    a fixed-count run of the image); the stopping rule (shift after I - 1
+    iterations > 1e-4 unless I = 1, reported shift = |c_I - c_(I-1)| and
+    <= 1e-4 unless I = 100); segments equal the CPU union-find of the labels
+    and its dense rank."""
+    torch = torch_cuda
+    ws = W.batch_workloads()
+    say(f"full size batch_256: generating {len(ws)} images")
+    planes = [torch.empty((3, w.n), dtype=torch.float32, device="cuda") for w in ws]
+    sa.gen_images([w.image for w in ws], planes)
+    imgs = []
+    for w, P in zip(ws, planes):
+        C = torch.empty((3, w.k), dtype=torch.float32, device="cuda")
+        sa.uniform_random_initialize(P, C, w.init_seed)
+        imgs.append(dict(points=P, width=w.image.width, height=w.image.height, centroids=C,
+                         init=C.clone(),
+                         labels=torch.full((w.n,), -7, dtype=torch.int32, device="cuda"),
+                         segments=torch.full((w.n,), -7, dtype=torch.int32, device="cuda")))
+    say(f"full size batch_256: one segment_batch call over "
+        f"{sum(w.n for w in ws) / 1e6:.1f} Mpx")
+    res = sa.segment_batch(imgs, max_iter=100, threshold=1e-4, connectivity=8)
+    torch.cuda.synchronize()
+    iters = np.array([r.iterations for r in res])
+    assert len(set(iters.tolist())) > 1
+    ctemp = torch.empty(sa.lib().split_hip_cluster_workspace(max(w.n for w in ws), 64, 3),
+                        dtype=torch.uint8, device="cuda")
+    for i, (w, im, r) in enumerate(zip(ws, imgs, res)):
+        if i % 32 == 0:
+            say(f"full size batch_256: checking images {i}..{min(i + 31, len(ws) - 1)}")
+        wd, ht = im["width"], im["height"]
+        tag = f"image {i} {wd}x{ht} k={w.k} I={r.iterations}"
+        assert 1 <= r.iterations <= 100, tag
+        # the centroids after I - 1 iterations, from a fixed-count run
+        Cp = im["init"].clone()
+        Lp = torch.empty(w.n, dtype=torch.int32, device="cuda")
+        if r.iterations > 1:
+            itp, shp = sa.cluster(im["points"], Cp, Lp, r.iterations - 1, -1.0, temp=ctemp,
+                                  width=wd, height=ht)
+            assert itp == r.iterations - 1 and shp > 1e-4, tag
+        dense = torch.empty_like(Lp)
+        sa.label_points(Cp, im["points"], dense, width=wd, height=ht)
+        assert torch.equal(dense, im["labels"]), tag
+        pts = im["points"].cpu().numpy()
+        lab = im["labels"].cpu().numpy()
+        cen = im["centroids"].cpu().numpy()
+        means, cnt = _fp64_means(pts, lab, w.k)
+        live = cnt > 0
+        assert np.abs(cen[:, live].astype(np.float64) - means[:, live]).max() <= CENT_TOL, tag
+        step = np.sqrt(((cen.astype(np.float64) - Cp.cpu().numpy()) ** 2).sum())
+        assert abs(step - r.shift) <= 1e-6 * max(1.0, step), tag
+        assert r.shift <= 1e-4 or r.iterations == 100, tag
+        rep, ncomp = oracle.cc_uf(lab.reshape(ht, wd), 8)
+        want, ns = oracle.compress(rep)
+        assert r.num_segments == ns == ncomp, tag
+        assert np.array_equal(im["segments"].cpu().numpy(), want), tag
