@@ -10,16 +10,26 @@ constexpr int kCcTile = 32;
 constexpr int kCcTilePx = kCcTile * kCcTile;
 constexpr int kCcThreads = 256;
 
-// Every access to the global forest after the tile pass is a device-scope
-// atomic load or store: the forest is rewritten in place by several
-// successive kernels (merge, jump, flatten, check), each of whose blocks may
-// run on any XCD, and a value must never be served from, or left dirty in,
-// one XCD's cache after a later kernel has replaced it.
+// Accesses to the global forest (the output array).  Loads are plain:
+// every value a forest word ever holds is an ancestor of its pixel (links
+// only point to larger indices and only ever move up the tree), so a load
+// that returns an older value still leads to the same root.  Stores are
+// plain only where one thread per kernel writes the word (the tile pass,
+// flatten: a pixel's own entry).  The root-jump pass is the one pass in
+// which two threads store to the same word (a walk's path-halving store
+// and the word's own final store); those stores are device atomics
+// (g_raise), performed at the coherence point and monotone, so the larger
+// value — the one nearer the root — is what every later kernel reads.
+// With plain racing stores the 2048^2 spiral ended with tile roots holding
+// the halving store's value after later kernels had written their root.
 __device__ __forceinline__ int g_load(const int32_t* par, int64_t i) {
-  return __hip_atomic_load(par + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return par[i];
 }
 __device__ __forceinline__ void g_store(int32_t* par, int64_t i, int v) {
-  __hip_atomic_store(par + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  par[i] = v;
+}
+__device__ __forceinline__ void g_raise(int32_t* par, int64_t i, int v) {
+  atomicMax(par + i, v);
 }
 
 // ---------------------------------------------------------- LDS forest ---
@@ -217,10 +227,10 @@ __device__ __forceinline__ void ccl_jump_one(int32_t* out, int64_t p, int W) {
       x = px2;
       break;
     }
-    g_store(out, x, gp);
+    g_raise(out, x, gp);
     x = gp;
   }
-  g_store(out, p, x);
+  g_raise(out, p, x);
 }
 
 // Follows the chain of pixel p to its root and writes it back.
