@@ -59,6 +59,7 @@ int segments_async(const int32_t* labels, int W, int H, int32_t* segments,
                    hipStream_t stream, CompressState** st_out);
 int compress_labels_finish(CompressState* st, int32_t* num_unique,
                            hipStream_t stream);
+size_t segments_workspace(int W, int H, bool compress);
 
 }  // namespace splithip
 
@@ -84,9 +85,8 @@ const char* split_hip_version(void) { return "0.1.0"; }
 
 size_t split_hip_segment_workspace(int32_t width, int32_t height, int32_t k) {
   const int64_t n = (int64_t)width * height;
-  return std::max({split_hip_cluster_workspace(n, k, 3),
-                   split_hip_connected_components_workspace(width, height),
-                   split_hip_compress_labels_workspace(n, n)});
+  return std::max(split_hip_cluster_workspace(n, k, 3),
+                  segments_workspace(width, height, true));
 }
 
 int split_hip_segment(const float* points, int32_t width, int32_t height,

@@ -4,20 +4,33 @@
 // over horizontal then vertical same-label runs
 // (split/src/split/device/ccl/connected_components.cu:80-181), which needs
 // about one iteration per turn of a spiral and a device->host read per
-// iteration.  Here a three-kernel union-find produces the same fixed point —
-// every pixel labelled with the LARGEST linear index in its component — in a
-// bounded number of passes:
+// iteration.  Here a union-find produces the same fixed point — every pixel
+// labelled with the LARGEST linear index in its component — in a fixed
+// number of passes over 64x64 tiles:
 //
-//  1. ccl_tile:   one 256-thread block per 32x32 tile; labels staged in LDS,
-//                 lock-free union-find in LDS (atomicMax links a smaller root
-//                 under a larger one, so a root is its tree's largest index),
-//                 then each pixel's global root index is written to `out`.
-//  2. ccl_merge:  pixels on tile edges union with equal-label neighbours in
-//                 other tiles, directly on the `out` forest (global atomicMax
-//                 with retry, as in Playne & Hawick's block-merge scheme).
-//  3. ccl_flatten: every pixel follows its chain to the root.
-//
-// The parent array is the output buffer itself, so no temp is needed.
+//  1. cc_tile:  one 1024-thread block per tile (wave w: rows w, w+16, w+32,
+//               w+48; lane = column).  Each pixel links to the end of its
+//               row run (ballot), vertical/diagonal unions between runs in
+//               an LDS forest (atomicMax puts the smaller root under the
+//               larger, so a root is the largest index of its tree), one
+//               find per run; every pixel writes its local root's global
+//               index into `out`.  The tile's left and right columns
+//               (labels and local roots) go to compact edge arrays, so the
+//               merge pass reads no image columns.
+//  2. cc_merge: the tile seams: top row against the row above, left column
+//               against the left tile's right column, and (8-connectivity)
+//               the up-right diagonal of the right column; unions on the
+//               global forest, deduplicated along runs.  A root that a union
+//               links under another is marked in the linked-root bitmap.
+//  3. cc_jump:  every marked root walks its chain to the root (path
+//               halving) and points at it.  The walks' stores are the only
+//               racing stores on the forest and are device atomicMax.
+//  4. cc_scan:  (with compression) roots = local roots not linked; popcount
+//               prefix of the root bitmap.
+//  5. cc_final: per tile, from the tile's own forest words staged in LDS: a
+//               pixel's root is its word (a root or a jumped root), or the
+//               word of its in-tile local root; writes the root index, or its
+//               rank among the roots (dense labels, compress_labels).
 //
 // compress_labels replaces copy/sort/unique/lower_bound
 // (split/src/split/device/ccl/compress_labels.cu:10-35) with a presence
@@ -25,61 +38,261 @@
 // dense rank, computed without sorting.
 #include <limits.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "ccl_device.hpp"
 
 namespace splithip {
 
+// Tile side S of the single-image CC (kCcSide).  Thread t
+// of a block handles tile pixels t + (S*S/4) j, j = 0..3: with S = 64 a
+// wave holds one tile row per j, with S = 32 two (lanes 0-31, 32-63).
+template <int S>
+struct CcTile {
+  static constexpr int kPx = S * S;
+  static constexpr int kThreads = kPx / 4;
+  static constexpr int kRowsPerPass = kThreads / S;
+};
 
-template <int CONN>
-__global__ __launch_bounds__(kCcThreads) void ccl_tile(
-    const int32_t* __restrict__ labels, int W, int H, int32_t* out,
-    unsigned* flag) {
-  __shared__ int lab[kCcTilePx];
-  __shared__ int par[kCcTilePx];
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *flag = 0u;
-  ccl_tile_body<CONN>(labels, W, H, out, blockIdx.x, blockIdx.y, lab, par);
+// Inputs, outputs and scratch of one connected-components run.
+struct CcArgs {
+  const int32_t* labels;
+  int32_t* out;   // the forest, then the result
+  int W, H, ntx;
+  unsigned* linked;      // bit p: root p was linked under another (N bits)
+  unsigned* local_root;  // bit p: p is its tile's local root (compression)
+  // per tile t, rows 0..S-1: labels and local roots of the left (x0) and
+  // right (x0 + S - 1) columns
+  int32_t* llab;
+  int32_t* rlab;
+  int32_t* lroot;
+  int32_t* rroot;
+};
+
+// ORs `bits` (the first `count` <= 64 bits used) into the bitmap at bit
+// offset p0 (any alignment); lanes k0, k0 + 1, k0 + 2 write one word each.
+__device__ __forceinline__ void or_bits(unsigned* bm, int64_t p0,
+                                        uint64_t bits, int lane, int k0) {
+  const int k = lane - k0;
+  if (bits == 0 || k < 0 || k > 2) return;
+  const int s = (int)(p0 & 31);
+  // word w: bits 0..31-s, word w+1: the next 32, word w+2: the last s
+  const unsigned v = k == 0   ? (unsigned)(bits << s)
+                     : k == 1 ? (unsigned)(bits >> (32 - s))
+                              : (unsigned)((bits >> (32 - s)) >> 32);
+  if (v) atomicOr(bm + (p0 >> 5) + k, v);
 }
 
-template <int CONN>
-__global__ __launch_bounds__(3 * kCcTile) void ccl_merge(
-    const int32_t* __restrict__ labels, int W, int H, int32_t* out) {
-  ccl_merge_body<CONN>(labels, W, H, out, blockIdx.x, blockIdx.y, threadIdx.x);
-}
-
-// Every pixel follows its chain to the root.  With `roots` non-null, wave w
-// also writes the presence bitmap of roots for its 64 pixels (bit p set iff
-// pixel p is a root), which is the value set compress_labels needs.
-__global__ void ccl_jump(int32_t* out, int64_t n, int W) {
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * blockDim.x)
-    ccl_jump_one(out, p, W);
-}
-
-template <int CONN>
-__global__ void ccl_check(const int32_t* __restrict__ labels, int W, int H,
-                          int32_t* out, unsigned* flag) {
-  const int64_t n = (int64_t)W * H;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * blockDim.x)
-    ccl_check_one<CONN>(labels, W, H, out, p, flag);
-}
-
-// flag == null: always; otherwise only when the check pass found a failure.
-__global__ void ccl_flatten(int32_t* out, int64_t n, uint32_t* roots,
-                            const unsigned* flag) {
-  if (flag && *flag == 0u) return;
+template <int CONN, int S>
+__global__ __launch_bounds__(CcTile<S>::kThreads) void cc_tile(CcArgs a) {
+  using G = CcTile<S>;
+  __shared__ int lab[G::kPx];
+  __shared__ int par[G::kPx];
+  const int tx = blockIdx.x, ty = blockIdx.y;
+  const int x0 = tx * S, y0 = ty * S;
+  const int tw = min(S, a.W - x0), th = min(S, a.H - y0);
   const int lane = threadIdx.x & (kWave - 1);
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = base + threadIdx.x;
-    const bool root = p < n && ccl_flatten_one(out, p) == p;
-    const uint64_t m = ballot(root);
-    if (roots && lane == 0 && p < n) {
-      const int64_t w = p >> 5;  // p (lane 0) is a multiple of 64
-      *reinterpret_cast<uint2*>(roots + w) =
-          make_uint2((uint32_t)m, (uint32_t)(m >> 32));
+  const int col = threadIdx.x & (S - 1);
+  const int rowbase = lane - col;  // first lane of this lane's row
+  const int64_t T = (int64_t)ty * a.ntx + tx;
+  int l[4], end[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    const bool in = col < tw && r < th;
+    l[j] = in ? a.labels[(int64_t)(y0 + r) * a.W + x0 + col] : INT_MIN;
+    lab[r * S + col] = l[j];
+  }
+  // each pixel's parent: the last pixel of its row run
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    const bool in = col < tw && r < th;
+    const int right = __shfl_down(l[j], 1);
+    const uint64_t m = ballot(in && col + 1 < tw && right == l[j]);
+    // bit S-1 of the row's bits is clear, so the shifted complement is
+    // nonzero below it
+    end[j] = col + (int)__builtin_ctzll(~(m >> lane));
+    par[r * S + col] = r * S + (in ? end[j] : col);
+  }
+  __syncthreads();
+  // unions between runs of consecutive rows, once per run overlap
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    const int left = __shfl_up(l[j], 1);
+    if (r == 0 || r >= th || col >= tw) continue;
+    const int li = r * S + col, c = l[j], up = li - S;
+    const bool left_same = col > 0 && left == c;
+    const int lu = lab[up];
+    const int lul = col > 0 ? lab[up - 1] : INT_MIN;
+    const bool up_same = lu == c;
+    if (up_same && !(left_same && lul == c)) lds_union(par, li, up);
+    if (CONN == 8) {
+      if (col > 0 && !left_same && !up_same && lul == c)
+        lds_union(par, li, up - 1);
+      if (col + 1 < tw && !up_same && lab[up + 1] == c)
+        lds_union(par, li, up + 1);
     }
+  }
+  __syncthreads();
+  // roots: one find per run (at its end), shared along the run
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    const bool in = col < tw && r < th;
+    const int li = r * S + col;
+    const int found = (in && col == end[j]) ? lds_find(par, li) : 0;
+    const int rl = __shfl(found, rowbase + (in ? end[j] : col));
+    const int g = (y0 + rl / S) * a.W + x0 + (rl & (S - 1));
+    const int64_t p = (int64_t)(y0 + r) * a.W + x0 + col;
+    if (in) g_store(a.out, p, g);
+    if (a.local_root) {
+      const uint64_t roots = ballot(in && rl == li);
+      if (r < th)
+        or_bits(a.local_root, (int64_t)(y0 + r) * a.W + x0,
+                S == 64 ? roots : (roots >> rowbase) & 0xFFFFFFFFull, lane,
+                rowbase);
+    }
+    if (in && col == 0) {
+      a.llab[T * S + r] = l[j];
+      a.lroot[T * S + r] = g;
+    }
+    if (in && col == S - 1) {
+      a.rlab[T * S + r] = l[j];
+      a.rroot[T * S + r] = g;
+    }
+  }
+}
+
+// One block of three groups of S threads per tile: group 0 the top row (up,
+// up-left, up-right neighbours in the rows above), group 1 the left column
+// (left and up-left neighbours in the left tile, from the edge arrays),
+// group 2 the right column's up-right neighbour in the right tile
+// (8-connectivity).  A union is skipped where the neighbouring seam pixel
+// already joins the same two runs, which leaves about one union per pair of
+// touching runs.
+template <int CONN, int S>
+__global__ __launch_bounds__(3 * S) void cc_merge(CcArgs a) {
+  const int tx = blockIdx.x, ty = blockIdx.y;
+  const int x0 = tx * S, y0 = ty * S;
+  const int tw = min(S, a.W - x0), th = min(S, a.H - y0);
+  const int t = threadIdx.x % S;
+  const int grp = threadIdx.x / S;
+  const int64_t T = (int64_t)ty * a.ntx + tx;
+  const int W = a.W;
+  if (grp == 0) {
+    if (y0 == 0 || t >= tw) return;
+    const int x = x0 + t;
+    const int p = y0 * W + x, u = p - W;
+    const int c = a.labels[p];
+    const int lu = a.labels[u];
+    const bool left_same = t > 0 && a.labels[p - 1] == c;
+    const int lul = x > 0 ? a.labels[u - 1] : INT_MIN;
+    if (lu == c && !(left_same && lul == c)) g_union(a.out, p, u, a.linked);
+    if (CONN == 8) {
+      if (x > 0 && lu != c && !left_same && lul == c)
+        g_union(a.out, p, u - 1, a.linked);
+      if (x + 1 < W && lu != c && a.labels[u + 1] == c)
+        g_union(a.out, p, u + 1, a.linked);
+    }
+  } else if (grp == 1) {
+    if (x0 == 0 || t >= th) return;
+    const int64_t e = T * S + t, el = (T - 1) * S + t;
+    const int c = a.llab[e];
+    const int lq = a.rlab[el];
+    const bool up_in = t > 0;
+    const bool up_same = up_in && a.llab[e - 1] == c;
+    const int ul = up_in ? a.rlab[el - 1] : INT_MIN;
+    if (lq == c && !(up_same && ul == c))
+      g_union(a.out, a.lroot[e], a.rroot[el], a.linked);
+    if (CONN == 8 && up_in && lq != c && !up_same && ul == c)
+      g_union(a.out, a.lroot[e], a.rroot[el - 1], a.linked);
+  } else if (CONN == 8) {
+    if (tw < S || x0 + S >= W || t == 0 || t >= th) return;
+    const int64_t e = T * S + t, er = (T + 1) * S + t;
+    const int c = a.rlab[e];
+    if (a.llab[er - 1] == c && a.rlab[e - 1] != c && a.llab[er] != c)
+      g_union(a.out, a.rroot[e], a.lroot[er - 1], a.linked);
+  }
+}
+
+// Every root that a union linked under another walks its chain to the root
+// (path halving) and points at it.  Chains run through linked roots only.
+__global__ __launch_bounds__(256) void cc_jump(int32_t* out,
+                                               const unsigned* linked,
+                                               int64_t words) {
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    unsigned m = linked[w];
+    while (m) {
+      const int a = (int)(w * 32 + __builtin_ctz(m));
+      m &= m - 1;
+      int x = g_load(out, a);
+      for (;;) {
+        const int px = g_load(out, x);
+        if (px == x) break;
+        const int gp = g_load(out, px);
+        if (gp == px) {
+          x = px;
+          break;
+        }
+        g_raise(out, x, gp);
+        x = gp;
+      }
+      g_raise(out, a, x);
+    }
+  }
+}
+
+// Bitmap lookup of a root's dense rank: chunk sum + word prefix + the lower
+// bits of its word (see bitmap_scan).
+__device__ __forceinline__ int32_t root_rank(int root, const unsigned* rb,
+                                             const unsigned* prefix,
+                                             const unsigned* csum);
+
+// Per tile: every pixel's root, from the tile's forest words staged in LDS.
+// After the jump pass a pixel's word holds itself (an unlinked local root:
+// a root), the root (a linked local root, jumped), or its local root in the
+// same tile (any other pixel; the tile pass wrote it and nothing rewrote it),
+// whose own word is then the root.  So the root is the word of the pixel's
+// word when that lies in the tile, else the pixel's word.  Writes the root
+// index (rb == null) or its dense rank among the roots.
+template <int S>
+__global__ __launch_bounds__(CcTile<S>::kThreads) void cc_final(
+    CcArgs a, const unsigned* rb, const unsigned* prefix,
+    const unsigned* csum) {
+  using G = CcTile<S>;
+  __shared__ int sv[G::kPx];
+  const int tx = blockIdx.x, ty = blockIdx.y;
+  const int x0 = tx * S, y0 = ty * S;
+  const int tw = min(S, a.W - x0), th = min(S, a.H - y0);
+  const int col = threadIdx.x & (S - 1);
+  const double inv_w = 1.0 / (double)a.W;
+  int v[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    const bool in = col < tw && r < th;
+    v[j] = in ? g_load(a.out, (int64_t)(y0 + r) * a.W + x0 + col) : 0;
+    sv[r * S + col] = v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    if (col >= tw || r >= th) continue;
+    const int64_t p = (int64_t)(y0 + r) * a.W + x0 + col;
+    const int v1 = v[j];
+    // row and column of v1 (double division, corrected: exact for v1 < 2^31)
+    int vy = (int)((double)v1 * inv_w);
+    if ((int64_t)vy * a.W > v1) --vy;
+    else if ((int64_t)(vy + 1) * a.W <= v1) ++vy;
+    const int vx = v1 - vy * a.W;
+    const bool in_tile = vy >= y0 && vy < y0 + th && vx >= x0 && vx < x0 + tw;
+    const int root = in_tile ? sv[(vy - y0) * S + (vx - x0)] : v1;
+    g_store(a.out, p, rb ? root_rank(root, rb, prefix, csum) : root);
   }
 }
 
@@ -92,14 +305,11 @@ struct CompressState {
   int err;          // 1: value span exceeds the bitmap capacity of temp
   int64_t words;    // bitmap words in use
   int64_t total;    // distinct values
-  unsigned ccl_flag;  // connected-component check failures (segment path)
 };
 
 constexpr size_t kCompressHead = 64;
-constexpr size_t kCcTempBytes = 256;  // the check pass's failure flag
 constexpr int kScanThreads = 256;
-constexpr int kScanWordsPerThread = 16;
-constexpr int kScanChunk = kScanThreads * kScanWordsPerThread;  // 4096 words
+constexpr int kScanChunk = 4 * kScanThreads;  // 1024 words
 
 __global__ void compress_init(CompressState* st) {
   st->lo = INT_MAX;
@@ -215,61 +425,66 @@ __global__ void compress_mark(const int32_t* v, int64_t n, int vec,
   }
 }
 
-// Value set known to be the roots of a connected-component labelling of n
-// pixels: values lie in [0, n) and the bitmap was written by ccl_flatten.
-__global__ void compress_init_roots(CompressState* st, int64_t n) {
-  st->lo = 0;
-  st->hi = (int)(n - 1);
-  st->arrive = 0u;
-  st->err = 0;
-  st->words = (n + 31) / 32;
-  st->total = 0;
-}
-
-// Per 4096-word chunk: exclusive popcount prefix of each word inside the
-// chunk, the chunk total into csum[]; the last block scans csum[] in place.
-__global__ __launch_bounds__(kScanThreads) void compress_scan(
-    CompressState* st, const unsigned* bm, unsigned* prefix, unsigned* csum) {
+// Exclusive popcount prefix of a bitmap of st->words words: per chunk of
+// kScanChunk words (256 threads x 4 consecutive words), each word's prefix
+// inside its chunk into prefix[], the chunk total into csum[] (chunk_scan
+// then scans csum[] in place and stores the total).  With `mask`,
+// the bitmap is first replaced by bm & ~mask (the roots: local roots not
+// linked).
+__global__ __launch_bounds__(kScanThreads) void bitmap_scan(
+    CompressState* st, unsigned* bm, const unsigned* mask, unsigned* prefix,
+    unsigned* csum) {
   if (st->err) return;
   const int64_t words = st->words;
   const int64_t nchunks = (words + kScanChunk - 1) / kScanChunk;
-  __shared__ unsigned s[kScanThreads];
-  __shared__ unsigned ticket;
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  __shared__ unsigned wsum[kScanThreads / kWave];
   for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    const int64_t w0 = c * kScanChunk + (int64_t)threadIdx.x * kScanWordsPerThread;
-    unsigned cnt[kScanWordsPerThread];
-    unsigned tsum = 0;
+    const int64_t w0 = c * kScanChunk + 4 * (int64_t)threadIdx.x;
+    unsigned v[4], cnt[4], tsum = 0;
 #pragma unroll
-    for (int j = 0; j < kScanWordsPerThread; ++j) {
-      const int64_t w = w0 + j;
-      cnt[j] = w < words ? __popc(bm[w]) : 0u;
-      tsum += cnt[j];
+    for (int i = 0; i < 4; ++i) {
+      const int64_t w = w0 + i;
+      v[i] = w < words ? bm[w] : 0u;
+      if (mask && w < words) {
+        v[i] &= ~mask[w];
+        bm[w] = v[i];
+      }
+      cnt[i] = __popc(v[i]);
+      tsum += cnt[i];
     }
-    s[threadIdx.x] = tsum;
+    // inclusive wave scan of the thread sums, then the waves' totals
+    unsigned inc = tsum;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const unsigned y = __shfl_up(inc, o);
+      inc += lane >= o ? y : 0u;
+    }
+    if (lane == kWave - 1) wsum[wv] = inc;
     __syncthreads();
-    for (int off = 1; off < kScanThreads; off <<= 1) {
-      const unsigned add = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
-      __syncthreads();
-      s[threadIdx.x] += add;
-      __syncthreads();
-    }
-    unsigned run = s[threadIdx.x] - tsum;  // exclusive
+    unsigned before = 0, total = 0;
 #pragma unroll
-    for (int j = 0; j < kScanWordsPerThread; ++j) {
-      const int64_t w = w0 + j;
-      if (w < words) prefix[w] = run;
-      run += cnt[j];
+    for (int k = 0; k < kScanThreads / kWave; ++k) {
+      before += k < wv ? wsum[k] : 0u;
+      total += wsum[k];
     }
-    if (threadIdx.x == kScanThreads - 1) csum[c] = s[threadIdx.x];
+    unsigned run = before + inc - tsum;  // exclusive
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (w0 + i < words) prefix[w0 + i] = run;
+      run += cnt[i];
+    }
+    if (threadIdx.x == 0) csum[c] = total;
     __syncthreads();
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) ticket = atomicAdd(&st->arrive, 1u);
-  __syncthreads();
-  if (ticket != gridDim.x - 1) return;
-  __threadfence();
-  // Last block: exclusive scan of the chunk totals, in place.
+}
+
+// Exclusive scan of the chunk totals of bitmap_scan, in place (one block);
+// the total into st.
+__global__ __launch_bounds__(kScanThreads) void chunk_scan(CompressState* st,
+                                                           unsigned* csum) {
+  if (st->err) return;
+  const int64_t nchunks = (st->words + kScanChunk - 1) / kScanChunk;
   const int64_t per = (nchunks + kScanThreads - 1) / kScanThreads;
   const int64_t b0 = (int64_t)threadIdx.x * per;
   uint64_t tsum = 0;
@@ -289,10 +504,15 @@ __global__ __launch_bounds__(kScanThreads) void compress_scan(
     csum[j] = (unsigned)run;
     run += c;
   }
-  if (threadIdx.x == kScanThreads - 1) {
-    st->total = (int64_t)s64[threadIdx.x];
-    st->arrive = 0u;
-  }
+  if (threadIdx.x == kScanThreads - 1) st->total = (int64_t)s64[threadIdx.x];
+}
+
+__device__ __forceinline__ int32_t root_rank(int root, const unsigned* rb,
+                                             const unsigned* prefix,
+                                             const unsigned* csum) {
+  const uint32_t w = (uint32_t)root >> 5;
+  const unsigned below = rb[w] & ((1u << (root & 31)) - 1u);
+  return (int32_t)(csum[w / kScanChunk] + prefix[w] + __popc(below));
 }
 
 __global__ void compress_map(int32_t* v, int64_t n, const CompressState* st,
@@ -319,11 +539,77 @@ static size_t compress_csum_bytes(int64_t words) {
   return align_up(((words + kScanChunk - 1) / kScanChunk + 1) * 4, 16);
 }
 
-int connected_components_impl(const int32_t* labels, int W, int H,
-                              int32_t* out, int conn, hipStream_t stream,
-                              uint32_t* roots, unsigned* flag) {
-  if (!labels || !out || !flag)
-    return fail(SPLIT_HIP_ERR_ARGUMENT, "labels, out or temp is null");
+// Zeroes the linked-root (and local-root) bitmaps; with st, sets up the
+// state for the root scan (values are pixel indices in [0, n)).
+__global__ __launch_bounds__(256) void cc_clear(unsigned* bm, int64_t words,
+                                                CompressState* st, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+       i += (int64_t)gridDim.x * blockDim.x)
+    bm[i] = 0u;
+  if (st && blockIdx.x == 0 && threadIdx.x == 0) {
+    st->lo = 0;
+    st->hi = (int)(n - 1);
+    st->arrive = 0u;
+    st->err = 0;
+    st->words = (n + 31) / 32;
+    st->total = 0;
+  }
+}
+
+// Temp block of connected_components: [state][linked-root bitmap][local-
+// root bitmap, word prefix, chunk sums: with compression][edge arrays].
+// Bitmaps carry 2 words after the last pixel's (or_bits64 may touch them).
+struct CcLayout {
+  CompressState* st;
+  unsigned* linked;
+  unsigned* local_root;
+  unsigned* prefix;
+  unsigned* csum;
+  int32_t* edge[4];
+  int64_t words;  // bitmap words incl. padding
+  size_t bytes;
+};
+
+// Tile side of the single-image CC passes.  Measured on the 4096^2 k=32
+// segment labels / the 8192^2 spiral: 64 -> 256 / 5,330 us for CC +
+// compress, 32 -> 280 / 9,840 us (four times the seams; longer root
+// chains for the jump pass).
+constexpr int kCcSide = 64;
+static int cc_side() { return kCcSide; }
+
+static CcLayout cc_layout(void* temp, int W, int H, bool compress) {
+  CcLayout L{};
+  const int S = cc_side();
+  const int64_t n = (int64_t)W * H;
+  const int64_t ntiles = (int64_t)((W + S - 1) / S) * ((H + S - 1) / S);
+  L.words = (n + 31) / 32 + 2;
+  char* t = static_cast<char*>(temp);
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    char* q = t ? t + o : nullptr;
+    o += align_up(bytes, 256);
+    return q;
+  };
+  L.st = reinterpret_cast<CompressState*>(take(sizeof(CompressState)));
+  // linked and local-root bitmaps adjacent: one clear covers both
+  L.linked = reinterpret_cast<unsigned*>(take((size_t)L.words * 4));
+  if (compress) {
+    L.local_root = L.linked ? L.linked + align_up((size_t)L.words * 4, 256) / 4
+                            : nullptr;
+    take((size_t)L.words * 4);
+    L.prefix = reinterpret_cast<unsigned*>(take((size_t)L.words * 4));
+    L.csum = reinterpret_cast<unsigned*>(take(compress_csum_bytes(L.words)));
+  }
+  for (int e = 0; e < 4; ++e)
+    L.edge[e] = reinterpret_cast<int32_t*>(take((size_t)ntiles * S * 4));
+  L.bytes = o;
+  return L;
+}
+
+static int cc_check_args(const int32_t* labels, int W, int H, int32_t* out,
+                         int conn) {
+  if (!labels || !out)
+    return fail(SPLIT_HIP_ERR_ARGUMENT, "labels or out is null");
   if (W < 1 || H < 1)
     return fail(SPLIT_HIP_ERR_ARGUMENT, "image is %d x %d", W, H);
   if ((int64_t)W * H > INT_MAX)
@@ -332,36 +618,63 @@ int connected_components_impl(const int32_t* labels, int W, int H,
   if (conn != 4 && conn != 8)
     return fail(SPLIT_HIP_ERR_ARGUMENT, "connectivity %d is not 4 or 8",
                 conn);
-  const int ntx = (W + kCcTile - 1) / kCcTile, nty = (H + kCcTile - 1) / kCcTile;
-  if (nty > 65535)
+  if ((H + cc_side() - 1) / cc_side() > 65535)
     return fail(SPLIT_HIP_ERR_UNSUPPORTED, "height %d too large", H);
-  const dim3 tiles(ntx, nty);
+  return SPLIT_HIP_OK;
+}
+
+// The pipeline of the header comment; with compress, out receives dense
+// ranks and L.st the count.
+static int cc_run(const int32_t* labels, int W, int H, int32_t* out, int conn,
+                  bool compress, const CcLayout& L, hipStream_t stream) {
   const int64_t n = (int64_t)W * H;
-  const dim3 pgrid(grid_for(n, 256, 8));
+  const int S = cc_side();
+  const int ntx = (W + S - 1) / S, nty = (H + S - 1) / S;
+  const dim3 tiles(ntx, nty);
+  CcArgs a{};
+  a.labels = labels;
+  a.out = out;
+  a.W = W;
+  a.H = H;
+  a.ntx = ntx;
+  a.linked = L.linked;
+  a.local_root = compress ? L.local_root : nullptr;
+  a.llab = L.edge[0];
+  a.rlab = L.edge[1];
+  a.lroot = L.edge[2];
+  a.rroot = L.edge[3];
+  const int64_t clear_words =
+      compress ? (int64_t)(L.local_root - L.linked) + L.words : L.words;
+  hipLaunchKernelGGL(cc_clear, dim3(grid_for(clear_words, 256, 4)), dim3(256),
+                     0, stream, L.linked, clear_words,
+                     compress ? L.st : nullptr, n);
+  constexpr int kS = kCcSide;
   if (conn == 8) {
-    hipLaunchKernelGGL(ccl_tile<8>, tiles, dim3(kCcThreads), 0, stream,
-                       labels, W, H, out, flag);
-    hipLaunchKernelGGL(ccl_merge<8>, tiles, dim3(3 * kCcTile), 0, stream,
-                       labels, W, H, out);
+    hipLaunchKernelGGL((cc_tile<8, kS>), tiles, dim3(CcTile<kS>::kThreads), 0,
+                       stream, a);
+    hipLaunchKernelGGL((cc_merge<8, kS>), tiles, dim3(3 * kS), 0, stream, a);
   } else {
-    hipLaunchKernelGGL(ccl_tile<4>, tiles, dim3(kCcThreads), 0, stream,
-                       labels, W, H, out, flag);
-    hipLaunchKernelGGL(ccl_merge<4>, tiles, dim3(3 * kCcTile), 0, stream,
-                       labels, W, H, out);
+    hipLaunchKernelGGL((cc_tile<4, kS>), tiles, dim3(CcTile<kS>::kThreads), 0,
+                       stream, a);
+    hipLaunchKernelGGL((cc_merge<4, kS>), tiles, dim3(3 * kS), 0, stream, a);
   }
-  SPLIT_HIP_CHECK_LAUNCH("ccl_tile/ccl_merge");
-  hipLaunchKernelGGL(ccl_jump, pgrid, dim3(256), 0, stream, out, n, W);
-  hipLaunchKernelGGL(ccl_flatten, pgrid, dim3(256), 0, stream, out, n, roots,
-                     (const unsigned*)nullptr);
-  if (conn == 8)
-    hipLaunchKernelGGL(ccl_check<8>, pgrid, dim3(256), 0, stream, labels, W,
-                       H, out, flag);
-  else
-    hipLaunchKernelGGL(ccl_check<4>, pgrid, dim3(256), 0, stream, labels, W,
-                       H, out, flag);
-  hipLaunchKernelGGL(ccl_flatten, pgrid, dim3(256), 0, stream, out, n, roots,
-                     (const unsigned*)flag);
-  SPLIT_HIP_CHECK_LAUNCH("ccl_flatten/ccl_check");
+  SPLIT_HIP_CHECK_LAUNCH("cc_tile/cc_merge");
+  const int64_t bits_words = (n + 31) / 32;
+  hipLaunchKernelGGL(cc_jump, dim3(grid_for(bits_words, 256, 8)), dim3(256), 0,
+                     stream, out, L.linked, bits_words);
+  if (compress) {
+    hipLaunchKernelGGL(bitmap_scan,
+                       dim3(grid_for((bits_words + kScanChunk - 1) / kScanChunk,
+                                     1, 8)),
+                       dim3(kScanThreads), 0, stream, L.st, L.local_root,
+                       L.linked, L.prefix, L.csum);
+    hipLaunchKernelGGL(chunk_scan, dim3(1), dim3(kScanThreads), 0, stream,
+                       L.st, L.csum);
+  }
+  hipLaunchKernelGGL(cc_final<kS>, tiles, dim3(CcTile<kS>::kThreads), 0,
+                     stream, a, compress ? L.local_root : nullptr, L.prefix,
+                     L.csum);
+  SPLIT_HIP_CHECK_LAUNCH("cc_jump/cc_final");
   return SPLIT_HIP_OK;
 }
 
@@ -406,8 +719,10 @@ static int compress_layout(void* temp, size_t temp_bytes, int64_t n,
 
 static int compress_rank(const CompressLayout& L, int32_t* labels, int64_t n,
                          hipStream_t stream) {
-  hipLaunchKernelGGL(compress_scan, dim3(grid_for(L.cap, kScanChunk, 1)),
-                     dim3(kScanThreads), 0, stream, L.st, L.bm, L.prefix,
+  hipLaunchKernelGGL(bitmap_scan, dim3(grid_for(L.cap, kScanChunk, 8)),
+                     dim3(kScanThreads), 0, stream, L.st, L.bm,
+                     (const unsigned*)nullptr, L.prefix, L.csum);
+  hipLaunchKernelGGL(chunk_scan, dim3(1), dim3(kScanThreads), 0, stream, L.st,
                      L.csum);
   hipLaunchKernelGGL(compress_map, dim3(grid_for(n, 256, 8)), dim3(256), 0,
                      stream, labels, n, L.st, L.bm, L.prefix, L.csum);
@@ -437,32 +752,26 @@ int compress_labels_async(int32_t* labels, int64_t n, void* temp,
   return SPLIT_HIP_OK;
 }
 
-// Connected components of `labels` into `segments`, then their dense ranks,
-// with the root bitmap written by the flatten pass (no min/max, clear or
-// mark pass).
+// Connected components of `labels` into `segments`, with compress their
+// dense ranks (the root bitmap comes from the CC passes themselves: no
+// min/max, clear or mark pass).
 int segments_async(const int32_t* labels, int W, int H, int32_t* segments,
                    int conn, bool compress, void* temp, size_t temp_bytes,
                    hipStream_t stream, CompressState** st_out) {
-  const int64_t n = (int64_t)W * H;
-  if (!compress) {
-    *st_out = nullptr;
-    return connected_components_impl(labels, W, H, segments, conn, stream,
-                                     nullptr, static_cast<unsigned*>(temp));
-  }
-  CompressLayout L;
-  SPLIT_HIP_TRY(compress_layout(temp, temp_bytes, n, &L));
-  const int64_t words = 2 * ((n + 63) / 64);
-  if (L.cap < words)
-    return fail(SPLIT_HIP_ERR_TEMP, "temp holds a bitmap of %lld words, %lld"
-                " needed", (long long)L.cap, (long long)words);
-  SPLIT_HIP_TRY(connected_components_impl(labels, W, H, segments, conn,
-                                          stream, L.bm, &L.st->ccl_flag));
-  hipLaunchKernelGGL(compress_init_roots, dim3(1), dim3(1), 0, stream, L.st,
-                     n);
-  SPLIT_HIP_CHECK_LAUNCH("compress_init_roots");
-  SPLIT_HIP_TRY(compress_rank(L, segments, n, stream));
-  *st_out = L.st;
+  SPLIT_HIP_TRY(cc_check_args(labels, W, H, segments, conn));
+  if (!temp) return fail(SPLIT_HIP_ERR_ARGUMENT, "temp is null");
+  const CcLayout L = cc_layout(temp, W, H, compress);
+  if (temp_bytes < L.bytes)
+    return fail(SPLIT_HIP_ERR_TEMP,
+                "temp holds %zu bytes, connected components need %zu",
+                temp_bytes, L.bytes);
+  SPLIT_HIP_TRY(cc_run(labels, W, H, segments, conn, compress, L, stream));
+  *st_out = compress ? L.st : nullptr;
   return SPLIT_HIP_OK;
+}
+
+size_t segments_workspace(int W, int H, bool compress) {
+  return cc_layout(nullptr, W, H, compress).bytes;
 }
 
 int compress_labels_finish(CompressState* st, int32_t* num_unique,
@@ -485,8 +794,10 @@ using namespace splithip;
 
 extern "C" {
 
-size_t split_hip_connected_components_workspace(int32_t, int32_t) {
-  return kCcTempBytes;
+size_t split_hip_connected_components_workspace(int32_t width,
+                                                int32_t height) {
+  if (width < 1 || height < 1) return 256;
+  return cc_layout(nullptr, width, height, false).bytes;
 }
 
 int split_hip_connected_components(const int32_t* labels, int32_t width,
@@ -494,15 +805,13 @@ int split_hip_connected_components(const int32_t* labels, int32_t width,
                                    size_t temp_bytes, int32_t* out, int32_t,
                                    int32_t connectivity,
                                    split_hip_stream_t stream) {
-  if (!labels || !out)
-    return fail(SPLIT_HIP_ERR_ARGUMENT, "labels or out is null");
-  if (!temp || temp_bytes < kCcTempBytes)
+  SPLIT_HIP_TRY(cc_check_args(labels, width, height, out, connectivity));
+  const CcLayout L = cc_layout(temp, width, height, false);
+  if (!temp || temp_bytes < L.bytes)
     return fail(SPLIT_HIP_ERR_TEMP,
                 "temp holds %zu bytes, connected_components needs %zu",
-                temp ? temp_bytes : (size_t)0, kCcTempBytes);
-  return connected_components_impl(labels, width, height, out, connectivity,
-                                   stream, nullptr,
-                                   static_cast<unsigned*>(temp));
+                temp ? temp_bytes : (size_t)0, L.bytes);
+  return cc_run(labels, width, height, out, connectivity, false, L, stream);
 }
 
 size_t split_hip_compress_labels_workspace(int64_t n, int64_t value_span) {

@@ -40,7 +40,17 @@ __device__ __forceinline__ void g_raise(int32_t* par, int64_t i, int v) {
 // still an ancestor, and unions only ever write roots (atomicMax, retried
 // when the root moved), so both kinds of write are safe together.
 
-__device__ __forceinline__ int lds_find(volatile int* par, int x) {
+// The forest lives in LDS; callers pass the __shared__ array, which these
+// helpers address as LDS explicitly (a generic pointer would compile to
+// flat instructions, which also wait on the vector-memory counter).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef volatile __attribute__((address_space(3))) int lds_int;
+#else
+typedef volatile int lds_int;
+#endif
+
+__device__ __forceinline__ int lds_find(int* par_generic, int x) {
+  lds_int* par = (lds_int*)par_generic;
   for (;;) {
     const int p = par[x];
     if (p == x) return x;
@@ -57,7 +67,9 @@ __device__ __forceinline__ void lds_union(int* par, int a, int b) {
     b = lds_find(par, b);
     if (a == b) return;
     if (a > b) { const int t = a; a = b; b = t; }
-    const int old = atomicMax(&par[a], b);
+    const int old = __hip_atomic_fetch_max((__attribute__((address_space(3))) int*)par + a, b,
+                                           __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old == a) return;
     a = old;
   }
@@ -140,14 +152,24 @@ __device__ __forceinline__ int g_find(const int* par, int i) {
   }
 }
 
-__device__ __forceinline__ void g_union(int* par, int a, int b) {
+// Joins the trees of a and b: the smaller root goes under the larger.  A
+// find may stop at a word's older value (another XCD's link not yet seen);
+// atomicMax returns the current value, and a root that turns out to be
+// linked already is retried from its current parent.  With `linked`, the
+// root that a successful link puts under another gets its bit set there
+// (the one thread whose atomicMax saw the root unlinked).
+__device__ __forceinline__ void g_union(int* par, int a, int b,
+                                        unsigned* linked = nullptr) {
   for (;;) {
     a = g_find(par, a);
     b = g_find(par, b);
     if (a == b) return;
     if (a > b) { const int t = a; a = b; b = t; }
     const int old = atomicMax(par + a, b);
-    if (old == a) return;
+    if (old == a) {
+      if (linked) atomicOr(linked + (a >> 5), 1u << (a & 31));
+      return;
+    }
     a = old;
   }
 }

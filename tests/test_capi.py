@@ -38,7 +38,11 @@ def test_host_only_entry_points_without_gpu():
     assert lib.split_hip_version().decode() == "0.1.0"
     assert lib.split_hip_status_string(2).decode() == "temp buffer too small"
     assert lib.split_hip_label_points_workspace(1000, 8) == 0
-    assert lib.split_hip_connected_components_workspace(64, 64) == 256
+    # CC: linked-root bitmap (N/32 words + 2) and the 64x64 tiles' edge
+    # arrays (4 x 64 ints per tile), each block 256-aligned, after a state slot
+    n = 4096 * 4096
+    ws = lib.split_hip_connected_components_workspace(4096, 4096)
+    assert n // 8 + 4 * 4096 * 256 <= ws < n // 8 + 4 * 4096 * 256 + 4096
     ws = lib.split_hip_cluster_workspace(4096 * 4096, 32, 3)
     n = 4096 * 4096
     # per-pixel Lloyd state (4 B) + interleaved pixel copy (16 B) + small tables
