@@ -27,6 +27,8 @@ struct TileArgs : Geometry {
   // bounded iterations of the single-image path: units (pairs of adjacent
   // tiles) and the multiplier of their spread order
   int64_t nunits, unit_mul;
+  // a tile with more failing pixels than this is re-assigned whole
+  int whole_fails;
 };
 
 // Per-pixel state: label | budget, where budget = slack + F[label] at the
