@@ -14,12 +14,18 @@ the slope of cluster() time between two fixed iteration counts (8 and 40).
 `traffic` is the HBM bytes per bounded tiles_step launch of the same
 iterations, from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) of
 `bench.py --profile-driver`, scaled by the counters of a 256 MiB copy.
-`roofline.valu_dense_equiv` puts the same iteration against the vector ALU:
-the dense assignment's 6 lane-ops per pixel and centroid over the measured
-time, as a fraction of the unpacked fp32 issue rate.
+`roofline.valu_issue` puts the same launches against the vector ALU: wave64
+VALU instructions issued per launch (a third counter pass, SQ_INSTS_VALU)
+over the launch's issue capacity (one per CU per cycle at 2.4 GHz).
 `cpu_baseline` times the oracle's restatement of the reference pipeline on a
 bounded crop of the same image on this host: on all OMP_NUM_THREADS threads
-(the oracle's OpenMP build; `cores`), and on one thread (`single_thread`).
+with the reference's host flags (`cores`), and on one thread of the checker
+build (`single_thread`).
+
+`--workload dropin_4096` times what a caller of the reference's C++ API gets
+on the same image: uniform_random_initialize, cluster() without an image
+width (the bounded path over rows of 16 points), connected_components and
+compress_labels as separate calls, each host-synchronous.
 
 `--workload batch_256` instead runs the batched path (256 images of uneven
 size and k, one graph-replayed launch sequence) against the same images
@@ -43,8 +49,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# unpacked fp32 vector issue: 256 CUs x 64 lanes x 2.4 GHz (lane-ops/s, 1e12)
-VALU_PEAK_TOPS = 39.32
+CLOCK_HZ = 2.4e9   # MI355X peak engine clock
+N_CU = 256
 
 
 def parse():
@@ -55,7 +61,9 @@ def parse():
     ap.add_argument("--workload", default="shaded_4096")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--cpu-crop", type=int, default=1024)
+    ap.add_argument("--cpu-crop", type=int, default=0,
+                    help="side of the CPU baseline's crop (default: 2048 for k <= 64, "
+                         "1024 above: about 10-30 s of CPU work)")
     ap.add_argument("--no-warm-start", action="store_true",
                     help="skip the learned warm-start comparison")
     ap.add_argument("--no-traffic", action="store_true",
@@ -122,7 +130,7 @@ def measure_traffic(workload):
         return None
     res = {}
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
-        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU"):
             rows = _counter_rows(exe, counter, workload, Path(d) / counter.lower())
             copy = [v for k, v in rows if "copyBuffer" in k]
             steps = [v for k, v in rows if k.startswith("void splithip::tiles_step")
@@ -132,11 +140,15 @@ def measure_traffic(workload):
             if not copy or len(steps) != want:
                 raise RuntimeError(f"{counter}: {len(copy)} copies, {len(steps)} bounded launches "
                                    f"(want {want})")
-            scale = COPY_BYTES / (max(copy) * 1024.0)
             sel = steps[ROOFLINE_LO - 1:ROOFLINE_HI - 1]   # iterations LO+1 .. HI
-            res[counter] = (sum(sel) / len(sel) * 1024.0 * scale, scale)
+            if counter == "SQ_INSTS_VALU":
+                res[counter] = sum(sel) / len(sel)
+            else:
+                scale = COPY_BYTES / (max(copy) * 1024.0)
+                res[counter] = (sum(sel) / len(sel) * 1024.0 * scale, scale)
     (rd, rs), (wr, ws) = res["FETCH_SIZE"], res["WRITE_SIZE"]
-    return {"read": rd, "write": wr, "read_scale": rs, "write_scale": ws}
+    return {"read": rd, "write": wr, "read_scale": rs, "write_scale": ws,
+            "valu_insts": res["SQ_INSTS_VALU"]}
 
 
 def batch_bench(args):
@@ -203,6 +215,63 @@ def batch_bench(args):
     same = all(a.iterations == b.iterations and a.num_segments == b.num_segments
                for a, b in zip(rb, rs))
     value = total / (ms_b / 1e3) / 1e6
+
+    # roofline of the batched bounded Lloyd step (batch_step_bounded<false>):
+    # every image runs a fixed count of iterations (threshold < 0, no graph),
+    # and the slope between 8 and 24 iterations is the time of one launch
+    # over all 256 images; algorithmic bytes: 12 B per pixel of the batch
+    def fixed(iters_):
+        cflat.copy_(init)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        sa.segment_batch(images, max_iter=iters_, threshold=-1.0,
+                         connectivity=w0.connectivity, use_graph=False, temp=temp)
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b)
+
+    fixed(8)
+    lo, hi = 8, 24
+    t_lo = min(fixed(lo) for _ in range(2))
+    t_hi = min(fixed(hi) for _ in range(2))
+    per_launch_ms = (t_hi - t_lo) / (hi - lo)
+    achieved = 12 * total / (per_launch_ms / 1e3) / 1e9
+    roof = {
+        "kernel": "batched Lloyd iteration: batch_step_bounded<false> over all 256 images "
+                  "(finalise each image, skip test, re-assign, exact sums)",
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "per_launch_us": round(per_launch_ms * 1e3, 2), "bytes_per_launch": 12 * total,
+        "timing": f"slope of segment_batch time between {lo} and {hi} fixed iterations "
+                  f"(HIP events, no graph)",
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        import numpy as np
+        import oracle
+        log("bench: cpu baseline (first images of the batch, until ~15 s)")
+        ref = oracle.lib_reference_flags()
+        done_px, spent, used = 0, 0.0, 0
+        for i, w in enumerate(ws):
+            if spent > 15.0:
+                break
+            img = images[i]["points"].cpu().numpy()
+            init_i = init[sum(x.k for x in ws[:i]) * 3:sum(x.k for x in ws[:i + 1]) * 3]
+            init_i = init_i.view(3, w.k).cpu().numpy()
+            dt, _ = _cpu_seconds(lambda: cpu_pipeline(img, w.image.width, w.image.height,
+                                                      init_i, rb[i].iterations, w.connectivity,
+                                                      handle=ref))
+            done_px += w.n
+            spent += dt
+            used += 1
+        cpu = {"value": round(done_px / spent / 1e6, 4), "unit": "Mpix/s",
+               "cores": oracle.threads(handle=ref), "kind": "port",
+               "sample": (f"the first {used} images of the batch ({done_px / 1e6:.1f} Mpix), "
+                          f"each with its GPU iteration count: the reference pipeline restated "
+                          f"(dense float distances, label-sorted reduce_by_key centroids, "
+                          f"union-find CC, sort/unique compress), oracle C with the reference's "
+                          f"flags, assignment threaded: {spent:.2f} s")}
     out = {
         "metric": "segmented Mpix/s (k-means+CC) & Lloyd iters to tol, warm-start vs plain",
         "value": round(value, 3),
@@ -225,13 +294,19 @@ def batch_bench(args):
             "stop": f"shift<{w0.threshold} or {w0.max_iter} iters",
             "connectivity": w0.connectivity,
             "lloyd_iters_mean": round(sum(r.iterations for r in rb) / len(rb), 2),
+            # sum over the images of pixels x iterations (the bounded launches
+            # of a run process sum(n_i (I_i - 1)) pixel-iterations)
+            "pixel_iterations": sum(w.n * r.iterations for w, r in zip(ws, rb)),
             "parallelism": "single GPU",
         },
         "separate_launches": {"ms_per_step": round(ms_s, 4),
                               "value": round(total / (ms_s / 1e3) / 1e6, 3),
                               "speedup_batched": round(ms_s / ms_b, 3),
                               "same_iterations_and_segments": same},
+        "roofline": roof,
     }
+    if cpu:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 
@@ -267,6 +342,41 @@ def cc_bench(args):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
     n = side * side
+    # the two calls apart: connected_components (cc_clear, cc_tile, cc_merge,
+    # cc_jump, cc_final: labels in, components out = 8 B/px algorithmic) and
+    # compress_labels (5 kernels, values in and ranks out = 8 B/px)
+    ms_cc = _events_ms(lambda: sa.connected_components(lab, out, connectivity=8,
+                                                       temp=cc_temp), args.steps, 1)
+    ms_cp = ms - ms_cc
+    roof = {
+        "kernel": "connected_components call: cc_clear, cc_tile<8,64>, cc_merge<8,64>, "
+                  "cc_jump, cc_final<64> (cc_jump dominates on this image: root chains "
+                  "along the spiral arms)",
+        "bound": "hbm",
+        "achieved": round(8 * n / (ms_cc / 1e3) / 1e9, 1),
+        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(8 * n / (ms_cc / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "per_call_us": round(ms_cc * 1e3, 1),
+        "bytes_per_call": 8 * n,
+        "compress_labels_call": {"per_call_us": round(ms_cp * 1e3, 1),
+                                 "achieved": round(8 * n / (ms_cp / 1e3) / 1e9, 1),
+                                 "frac": round(8 * n / (ms_cp / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        log("bench: cpu baseline (union-find CC + compress of the same labels)")
+        ref = oracle.lib_reference_flags()
+        host = lab.cpu().numpy()
+        dt, (rep_, _) = _cpu_seconds(lambda: oracle.cc_uf(host, 8, handle=ref))
+        dtc, _ = _cpu_seconds(lambda: oracle.compress(rep_, handle=ref))
+        cpu = {"value": round(n / (dt + dtc) / 1e6, 3), "unit": "Mpix/s", "cores": 1,
+               "kind": "port",
+               "sample": (f"the whole {side}x{side} label image: union-find CC (8-conn, the "
+                          f"reference's converged result) + sort/unique compress, oracle C "
+                          f"with the reference's flags, one thread (serial code): "
+                          f"{dt + dtc:.2f} s")}
     out_line = {
         "metric": "connected components + compress Mpix/s (8-conn, stress labels)",
         "value": round(n / (ms / 1e3) / 1e6, 3), "unit": "Mpix/s", "n_gpus": 1,
@@ -277,7 +387,10 @@ def cc_bench(args):
                    "connectivity": 8, "segments": nseg, "parallelism": "single GPU"},
         "algorithmic_bytes_per_step": 16 * n,
         "hbm_frac": round(16 * n / (ms / 1e3) / (HBM_PEAK_GBS * 1e9), 4),
+        "roofline": roof,
     }
+    if cpu:
+        out_line["cpu_baseline"] = cpu
     print(json.dumps(out_line), flush=True)
 
 
@@ -521,8 +634,30 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _cpu_seconds(fn):
+    t0 = time.perf_counter()
+    r = fn()
+    return time.perf_counter() - t0, r
+
+
+def cpu_pipeline(planes, width, height, init, iters, conn, handle=None):
+    """The reference pipeline restated on the CPU (oracle): iters Lloyd
+    iterations of dense float distances + label-sorted reduce_by_key
+    centroids, union-find CC, sort/unique compress; returns the labels."""
+    import oracle
+    _, labels, it, _ = oracle.ref_cluster(planes, init, iters, -1.0, handle=handle)
+    rep, _ = oracle.cc_uf(labels.reshape(height, width), conn, handle=handle)
+    oracle.compress(rep, handle=handle)
+    return labels
+
+
 def cpu_baseline(img, width, height, w, gpu_iters, crop):
-    """Reference pipeline restated on the CPU (oracle), on a crop."""
+    """Reference pipeline restated on the CPU (oracle) on a center crop, on
+    this host: all threads with the reference's host flags
+    (-Ofast -march=native -fopenmp, split's common.pri:27-32; OpenMP splits
+    the per-pixel assignment, as the reference's OMP Thrust backend would;
+    the counting sort, CC and compress run on one thread), and one thread
+    of the checker build (-O3, no fast-math)."""
     import numpy as np
     import oracle
     c = min(crop, width, height)
@@ -531,31 +666,25 @@ def cpu_baseline(img, width, height, w, gpu_iters, crop):
     sub = np.ascontiguousarray(planes.reshape(3, c * c))
     idx = oracle.sample_indices(w.init_seed, c * c, w.k)
     init = np.ascontiguousarray(sub[:, idx])
-    def run(parallel):
-        t0 = time.perf_counter()
-        cent, labels, it, _ = oracle.ref_cluster(sub, init, gpu_iters, -1.0,
-                                                 parallel=parallel)
-        rep, _ = oracle.cc_uf(labels.reshape(c, c), w.connectivity)
-        oracle.compress(rep)
-        return time.perf_counter() - t0, it, labels
-
-    dt1, it, lab1 = run(False)
-    # all cores: the OpenMP build of the same oracle (per-pixel assignment
-    # split over OMP_NUM_THREADS threads; identical results)
-    dtp, _, labp = run(True)
-    cores = oracle.threads(True)
-    if not np.array_equal(lab1, labp):
-        raise RuntimeError("OpenMP oracle build disagrees with the plain build")
+    ref = oracle.lib_reference_flags()
+    dt1, lab1 = _cpu_seconds(lambda: cpu_pipeline(sub, c, c, init, gpu_iters, w.connectivity))
+    dtp, labp = _cpu_seconds(lambda: cpu_pipeline(sub, c, c, init, gpu_iters, w.connectivity,
+                                                  handle=ref))
+    cores = oracle.threads(handle=ref)
+    agree = float(np.mean(lab1 == labp))
     return {
         "value": round(c * c / dtp / 1e6, 4),
         "unit": "Mpix/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"center {c}x{c} crop of the {w.name} image, k={w.k}, {it} Lloyd "
+        "sample": (f"center {c}x{c} crop of the {w.name} image, k={w.k}, {gpu_iters} Lloyd "
                    f"iterations of the reference algorithm (dense float distances, "
                    f"label-sorted reduce_by_key centroids) + union-find CC "
-                   f"({w.connectivity}-conn) + sort/unique compress; oracle C with OpenMP "
-                   f"over pixels, {cores} threads: {dtp:.2f} s; 1 thread: {dt1:.2f} s"),
+                   f"({w.connectivity}-conn) + sort/unique compress; oracle C built with the "
+                   f"reference's flags {' '.join(oracle.REFERENCE_FLAGS)}, {cores} threads "
+                   f"(assignment threaded, sort/CC/compress on one thread): {dtp:.2f} s; "
+                   f"checker build (-O3, no fast-math), 1 thread: {dt1:.2f} s; "
+                   f"labels agree on {agree * 100:.3f}% of pixels"),
         "single_thread": {"value": round(c * c / dt1 / 1e6, 4), "cores": 1,
                           "seconds": round(dt1, 2)},
     }
@@ -592,7 +721,8 @@ def main():
     if args.workload == "erode_4096":
         erode_bench(args)
         return
-    w = W.WORKLOADS[args.workload]
+    dropin = args.workload == "dropin_4096"
+    w = W.SHADED_4096 if dropin else W.WORKLOADS[args.workload]
     if args.profile_driver:
         profile_driver(w)
         return
@@ -612,6 +742,23 @@ def main():
         sa.uniform_random_initialize(P, C, w.init_seed)
         return sa.segment(P, width, height, C, L, S, max_iter=w.max_iter,
                           threshold=w.threshold, connectivity=w.connectivity, temp=temp)
+
+    if dropin:
+        # the reference's call sequence (samples/src/main.cu:228-253) through
+        # the C ABI the C++ drop-ins forward to, with their temp contract
+        dtemp = torch.empty(max(sa.lib().split_hip_cluster_workspace(n, w.k, 3),
+                                sa.lib().split_hip_connected_components_workspace(width,
+                                                                                  height),
+                                sa.lib().split_hip_compress_labels_workspace(n, n)),
+                            dtype=torch.uint8, device="cuda")
+
+        def step():
+            sa.uniform_random_initialize(P, C, w.init_seed)
+            it, shift = sa.cluster(P, C, L, w.max_iter, w.threshold, temp=dtemp)
+            sa.connected_components(L.view(height, width), S, connectivity=w.connectivity,
+                                    temp=dtemp)
+            ns = sa.compress_labels(S, temp=dtemp)
+            return sa.SegmentResult(it, shift, ns)
 
     for i in range(args.warmup):
         r = step()
@@ -646,7 +793,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded generator)",
         "config": {
-            "workload": w.name,
+            "workload": args.workload,
             "image": f"{width}x{height}",
             "k": w.k,
             "init": "seeded sample (plain, no warm start)",
@@ -672,7 +819,8 @@ def main():
             torch.cuda.synchronize()
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record(stream)
-            sa.cluster(P, Ct, Lt, iters_, -1.0, temp=ctemp, width=width, height=height)
+            geo = {} if dropin else dict(width=width, height=height)
+            sa.cluster(P, Ct, Lt, iters_, -1.0, temp=ctemp, **geo)
             b.record(stream)
             torch.cuda.synchronize()
             return a.elapsed_time(b)
@@ -684,8 +832,9 @@ def main():
         per_launch_ms = (t_hi - t_lo) / (hi - lo)
         bytes_per_launch = 12 * n
         achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
+        cpl = 1 if w.k <= 64 else 2 if w.k <= 128 else 4
         out["roofline"] = {
-            "kernel": "Lloyd iteration: tiles_step<1, 3, false> (finalise the previous "
+            "kernel": f"Lloyd iteration: tiles_step<{cpl}, 3, false> (finalise the previous "
                       "step, skip test, re-assign, exact sums)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -695,20 +844,12 @@ def main():
             "traffic": None,
             "per_launch_us": round(per_launch_ms * 1e3, 2),
             "bytes_per_launch": bytes_per_launch,
-            # the VALU side for k >= 32: the dense direct-form assignment's
-            # lane operations (3 sub + 1 mul + 2 fma per pixel and centroid)
-            # per iteration over the same time, against gfx950's unpacked
-            # fp32 issue rate; above 1 = the bounds skip more work than the
-            # dense form could do at peak
-            "valu_dense_equiv": {
-                "lane_ops": 6 * n * w.k,
-                "achieved_tops": round(6 * n * w.k / (per_launch_ms / 1e3) / 1e12, 2),
-                "peak_tops": VALU_PEAK_TOPS,
-                "frac": round(6 * n * w.k / (per_launch_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 3),
-            },
         }
         log(f"bench: lloyd_step {per_launch_ms * 1e3:.2f} us/launch -> {achieved:.1f} GB/s")
-        if not args.no_traffic:
+        if dropin:
+            out["roofline"]["traffic_note"] = ("counter passes run on the default "
+                                               "workload's image-tiled launches")
+        if not args.no_traffic and not dropin:
             log("bench: counter passes for roofline.traffic")
             try:
                 t = measure_traffic(w.name)
@@ -721,10 +862,19 @@ def main():
                 out["roofline"]["traffic_write"] = round(t["write"])
                 out["roofline"]["traffic_scale"] = [round(t["read_scale"], 4),
                                                     round(t["write_scale"], 4)]
+                # the vector ALU: wave64 VALU instructions issued per launch
+                # (SQ_INSTS_VALU) against one per CU per cycle (4 SIMDs, a
+                # wave64 instruction every 4 cycles each) over the launch
+                cap = per_launch_ms / 1e3 * CLOCK_HZ * N_CU
+                out["roofline"]["valu_issue"] = {
+                    "insts_per_launch": round(t["valu_insts"]),
+                    "capacity_per_launch": round(cap),
+                    "frac": round(t["valu_insts"] / cap, 4),
+                }
                 log(f"bench: traffic {t['read'] / 1e6:.1f} MB read + {t['write'] / 1e6:.1f} MB "
                     f"written per launch")
 
-    if not args.no_warm_start:
+    if not args.no_warm_start and args.workload == "shaded_4096":
         # The learned warm start against the plain method on its evaluation
         # set (256 config-2-style 1024x1024 images, k=16): plain separate
         # segment() calls, plain batched, warm-started batched.
@@ -746,7 +896,8 @@ def main():
 
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(img, width, height, w, iters[-1], args.cpu_crop)
+        crop = args.cpu_crop or (2048 if w.k <= 64 else 1024)
+        out["cpu_baseline"] = cpu_baseline(img, width, height, w, iters[-1], crop)
 
     print(json.dumps(out), flush=True)
 

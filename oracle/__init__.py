@@ -59,6 +59,29 @@ def _load(path: Path) -> ctypes.CDLL:
     return h
 
 
+_lib_ref = None
+
+# The reference's host flags (split's common.pri:27-32): the CPU baseline's
+# all-cores timing build.  -march=native must match the CPU that runs it, so
+# this build is made where it is used, into a temporary directory, and never
+# shipped.
+REFERENCE_FLAGS = ["-Ofast", "-march=native", "-fopenmp", "-mfma", "-mavx2"]
+
+
+def lib_reference_flags() -> ctypes.CDLL:
+    """oracle.c built with the reference's host flags and OpenMP, for timing
+    only (fast-math: not the checker)."""
+    global _lib_ref
+    if _lib_ref is None:
+        import tempfile
+        d = Path(tempfile.mkdtemp(prefix="oracle_ref_flags_"))
+        out = d / "liboracle_refflags.so"
+        subprocess.run(["gcc", *REFERENCE_FLAGS, "-fPIC", "-shared", "-std=c11",
+                        str(HERE / "oracle.c"), "-o", str(out), "-lm"], check=True)
+        _lib_ref = _load(out)
+    return _lib_ref
+
+
 def lib(parallel: bool = False) -> ctypes.CDLL:
     global _lib, _lib_omp
     if parallel:
@@ -114,17 +137,17 @@ def inertia(pts, cent, labels):
                                 cent.shape[1], labels.ctypes.data)
 
 
-def cc_uf(labels_hw, conn):
+def cc_uf(labels_hw, conn, handle=None):
     labels_hw = _i32(labels_hw)
     h, w = labels_hw.shape
     out = np.empty(h * w, np.int32)
-    cnt = lib().oracle_cc_uf(labels_hw.ctypes.data, w, h, int(conn), out.ctypes.data)
+    cnt = (handle or lib()).oracle_cc_uf(labels_hw.ctypes.data, w, h, int(conn), out.ctypes.data)
     return out, int(cnt)
 
 
-def compress(labels):
+def compress(labels, handle=None):
     a = _i32(labels).copy()
-    cnt = lib().oracle_compress(a.ctypes.data, a.size)
+    cnt = (handle or lib()).oracle_compress(a.ctypes.data, a.size)
     return a, int(cnt)
 
 
@@ -150,12 +173,13 @@ def ref_calculate_centroids(labels, pts, k):
     return cent.reshape(3, k)
 
 
-def ref_cluster(pts, init, max_iter, threshold, parallel=False):
+def ref_cluster(pts, init, max_iter, threshold, parallel=False, handle=None):
+    """handle: another build of the same code (the timing builds)."""
     pts = _f32(pts)
     cent = _f32(init).copy()
     labels = np.empty(pts.shape[1], np.int32)
     d = ctypes.c_float(0.0)
-    it = lib(parallel).ref_cluster(pts.ctypes.data, pts.shape[1], cent.ctypes.data,
+    it = (handle or lib(parallel)).ref_cluster(pts.ctypes.data, pts.shape[1], cent.ctypes.data,
                                    cent.shape[1], labels.ctypes.data, int(max_iter),
                                    float(threshold), ctypes.byref(d))
     return cent, labels, int(it), float(d.value)
@@ -181,6 +205,6 @@ def canonical_min(rep):
     return mins[rep].astype(np.int64)
 
 
-def threads(parallel=False):
+def threads(parallel=False, handle=None):
     """Threads the (OpenMP) build runs its per-pixel loops on."""
-    return int(lib(parallel).oracle_threads())
+    return int((handle or lib(parallel)).oracle_threads())
