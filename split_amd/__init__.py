@@ -385,9 +385,10 @@ def compress_labels(labels, temp=None) -> int:
     n = labels.numel()
     L = lib()
     if temp is None:
-        # size the bitmap for the actual value span (torch only reads min/max)
-        lo, hi = (int(x) for x in torch.aminmax(labels)) if n else (0, 0)
-        temp = _temp(max(L.split_hip_compress_labels_workspace(n, hi - lo + 1), 4 * n), labels)
+        # the reference's contract: a temp of N ints, whose bitmap holds a
+        # value span of 32 N (any connected-components output); a wider span
+        # returns SPLIT_HIP_ERR_RANGE, and a caller passes a larger temp
+        temp = _temp(max(L.split_hip_compress_labels_workspace(n, n), 4 * n), labels)
     cnt = _c_i32(0)
     _check(L.split_hip_compress_labels(li, n, temp.data_ptr(), temp.numel(),
                                        ctypes.byref(cnt), _stream()), "compress_labels")
