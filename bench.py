@@ -877,22 +877,35 @@ def main():
     if not args.no_warm_start and args.workload == "shaded_4096":
         # The learned warm start against the plain method on its evaluation
         # set (256 config-2-style 1024x1024 images, k=16): plain separate
-        # segment() calls, plain batched, warm-started batched.
+        # segment() calls, plain batched, the network's prediction, the
+        # prediction refined on the 256^2 coarse image (the shipped warm
+        # start) and, as the ablation, the seeded sample refined the same way.
         from split_amd import warmstart as WS
         log("bench: warm-start evaluation (256 x 1024^2, k=16)")
         ev = WS.evaluate(WS.WarmStartNet.load(), count=256, side=1024, separate=True,
                          log=log)["summary"]
-        out["warm_start"] = {
+        ws = {
             "workload": "256 x 1024x1024 shaded Voronoi, k=16, shift<1e-4 or 100 iters",
             "plain_separate_mpix_s": round(ev["separate_plain_mpix_s"], 1),
             "plain_batched_mpix_s": round(ev["plain_mpix_s"], 1),
-            "warm_batched_mpix_s": round(ev["warm_mpix_s"], 1),
-            "speedup_warm_batched_vs_plain_separate":
-                round(ev["speedup_batched_warm_vs_separate_plain"], 3),
             "plain_iters_mean": round(ev["plain_iters_mean"], 2),
-            "warm_iters_mean": round(ev["warm_iters_mean"], 2),
-            "inertia_ratio_warm_vs_plain": round(ev["inertia_ratio_total"], 4),
+            "plain_cap_hits": ev["plain_cap_hits"],
         }
+        for name in ("warm", "warm_coarse", "plain_coarse"):
+            ws.update({
+                f"{name}_batched_mpix_s": round(ev[f"{name}_mpix_s"], 1),
+                f"{name}_iters_mean": round(ev[f"{name}_iters_mean"], 2),
+                f"{name}_cap_hits": ev[f"{name}_cap_hits"],
+                f"{name}_inertia_ratio": round(ev[f"{name}_inertia_ratio_total"], 4),
+            })
+            if f"{name}_coarse_iters_mean" in ev:
+                ws[f"{name}_coarse_iters_mean"] = round(ev[f"{name}_coarse_iters_mean"], 2)
+                ws[f"{name}_pixel_iters_mean"] = round(ev[f"{name}_pixel_iters_mean"], 2)
+        ws["speedup_warm_coarse_batched_vs_plain_separate"] = round(
+            ev["speedup_batched_warm_coarse_vs_separate_plain"], 3)
+        ws["speedup_warm_coarse_vs_plain_batched"] = round(
+            ev["plain_ms"] / ev["warm_coarse_ms"], 3)
+        out["warm_start"] = ws
 
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")

@@ -242,7 +242,8 @@ typedef struct split_hip_image_desc {
   const float* points;  /* 3 SoA planes of width*height floats */
   float* centroids;     /* [3][k], in: initial, out: final */
   int32_t* labels;      /* width*height cluster labels */
-  int32_t* segments;    /* width*height segment labels */
+  int32_t* segments;    /* width*height segment labels (unused, may be NULL,
+                           with connectivity 0) */
   int32_t width, height;
   int32_t k;            /* 1..SPLIT_HIP_BATCH_MAX_K */
   int32_t reserved;
@@ -251,7 +252,7 @@ typedef struct split_hip_image_desc {
 typedef struct split_hip_batch_result {
   int32_t iterations;
   float shift;
-  int32_t num_segments; /* -1 when compress == 0 */
+  int32_t num_segments; /* -1 when compress == 0 or connectivity == 0 */
   int32_t reserved;
 } split_hip_batch_result;
 
@@ -261,8 +262,9 @@ typedef struct split_hip_batch_result {
 size_t split_hip_segment_batch_workspace(const split_hip_image_desc* images,
                                          int32_t count);
 /* Every image: clustering from its centroids (max_iter / threshold as
- * split_hip_cluster), connected components of its labels and, with
- * `compress`, compressed segment labels.  use_graph != 0 captures the
+ * split_hip_cluster), connected components of its labels (connectivity 4
+ * or 8; 0: the clustering alone) and, with `compress`, compressed segment
+ * labels.  use_graph != 0 captures the
  * launch sequence into a hipGraph on first use (cached per temp pointer and
  * batch shape) and replays it.  results (count entries) may be NULL.  The
  * call returns after the batch finished on `stream`. */
@@ -319,6 +321,13 @@ size_t split_hip_ws_workspace(int32_t batch);
 int split_hip_ws_pack(const split_hip_image_desc* images, int32_t count,
                       uint16_t* input, void* temp, size_t temp_bytes,
                       split_hip_stream_t stream);
+/* The same area-average resample as float SoA planes: `planes` = count x 3
+ * x 256^2 floats, the coarse images on which predicted centroids are
+ * refined before the full-size run.  temp holds count * 16 bytes.  Returns
+ * after the planes are written. */
+int split_hip_ws_coarse(const split_hip_image_desc* images, int32_t count,
+                        float* planes, void* temp, size_t temp_bytes,
+                        split_hip_stream_t stream);
 /* Forward pass over `batch` inputs: input slot b is image index[b] of
  * `input` (index NULL: slot b).  out (batch x 48 floats, may be NULL)
  * receives the sigmoid outputs.  Keeps the activations in ws. */

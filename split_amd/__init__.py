@@ -137,6 +137,8 @@ CABI = {
     "split_hip_ws_workspace": (_c_size_t, [_c_i32]),
     "split_hip_ws_pack": (ctypes.c_int, [ctypes.POINTER(ImageDesc), _c_i32, _c_void_p, _c_void_p,
                                          _c_size_t, _c_void_p]),
+    "split_hip_ws_coarse": (ctypes.c_int, [ctypes.POINTER(ImageDesc), _c_i32, _c_void_p,
+                                           _c_void_p, _c_size_t, _c_void_p]),
     "split_hip_ws_forward": (ctypes.c_int, [_c_void_p, _c_void_p, _c_void_p, _c_i32, _c_void_p,
                                             _c_size_t, _c_void_p, _c_void_p]),
     "split_hip_ws_backward": (ctypes.c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_i32,
@@ -627,7 +629,8 @@ def _batch_descs(images):
             raise ValueError(f"image {i}: points must hold width*height pixels")
         c, k = _cent(im["centroids"])
         lb = _dev(im["labels"], torch.int32, "labels")
-        sg = _dev(im["segments"], torch.int32, "segments")
+        seg = im.get("segments")
+        sg = None if seg is None else _dev(seg, torch.int32, "segments")
         descs[i] = ImageDesc(p, c, lb, sg, w, h, k, 0)
     return descs
 
@@ -644,7 +647,8 @@ def segment_batch(images, *, max_iter: int = 100, threshold: float = 1e-4,
     """segment() of every image in one batched, graph-replayed launch sequence.
 
     images: sequence of dicts with keys points (3, w*h), width, height,
-    centroids (3, k) in/out, labels (w*h,), segments (w*h,); k <= 64.
+    centroids (3, k) in/out, labels (w*h,), segments (w*h,; may be absent
+    with connectivity 0, the clustering alone); k <= 64.
     Each image's outputs equal segment() of that image alone."""
     descs = _batch_descs(images)
     L = lib()

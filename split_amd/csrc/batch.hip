@@ -661,7 +661,7 @@ struct BatchPlan {
 };
 
 static int make_plan(const split_hip_image_desc* d, int count, void* temp,
-                     int max_iter, BatchPlan* P) {
+                     bool need_segments, BatchPlan* P) {
   for (int i = 0; i < count; ++i) {
     const split_hip_image_desc& e = d[i];
     if (e.width < 1 || e.height < 1 || (int64_t)e.width * e.height > INT32_MAX)
@@ -670,10 +670,10 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
     if (e.k < 1 || e.k > kBatchMaxK)
       return fail(SPLIT_HIP_ERR_UNSUPPORTED, "image %d: k = %d (batch k <= %d)",
                   i, e.k, kBatchMaxK);
-    if (temp && (!e.points || !e.centroids || !e.labels || !e.segments))
+    if (temp && (!e.points || !e.centroids || !e.labels ||
+                 (need_segments && !e.segments)))
       return fail(SPLIT_HIP_ERR_ARGUMENT, "image %d has a null buffer", i);
   }
-  (void)max_iter;
   auto bounded = [&](int i) { return d[i].k >= kBoundedMinK; };
   P->order.clear();
   for (int i = 0; i < count; ++i)
@@ -853,6 +853,7 @@ static void enqueue_batch(const BatchArgs& a, const Groups& G, int conn,
     hipLaunchKernelGGL(batch_labels_state, dim3(8 * cus), dim3(256), 0, stream,
                        a);
   hipLaunchKernelGGL(batch_finish, dim3(a.count), dim3(256), 0, stream, a);
+  if (conn == 0) return;  // clustering only
   const int cgrid = (int)std::min<int64_t>(a.ctiles, (int64_t)cus * 16);
   if (conn == 8) {
     hipLaunchKernelGGL(batch_ccl_tile<8>, dim3(cgrid), dim3(kCcThreads), 0,
@@ -907,7 +908,7 @@ size_t split_hip_segment_batch_workspace(const split_hip_image_desc* images,
                                          int32_t count) {
   if (!images || count < 1) return 0;
   BatchPlan P;
-  if (make_plan(images, count, nullptr, 0, &P) != SPLIT_HIP_OK) return 0;
+  if (make_plan(images, count, nullptr, false, &P) != SPLIT_HIP_OK) return 0;
   return P.total_bytes;
 }
 
@@ -919,10 +920,12 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
                             split_hip_stream_t stream) {
   if (!images || count < 1 || !temp)
     return fail(SPLIT_HIP_ERR_ARGUMENT, "images, count or temp missing");
-  if (connectivity != 4 && connectivity != 8)
+  // connectivity 0: the clustering alone (centroids and labels; segments
+  // may be null and are not written)
+  if (connectivity != 0 && connectivity != 4 && connectivity != 8)
     return fail(SPLIT_HIP_ERR_ARGUMENT, "connectivity %d", connectivity);
   BatchPlan P;
-  SPLIT_HIP_TRY(make_plan(images, count, temp, max_iter, &P));
+  SPLIT_HIP_TRY(make_plan(images, count, temp, connectivity != 0, &P));
   if (temp_bytes < P.total_bytes)
     return fail(SPLIT_HIP_ERR_TEMP, "temp holds %zu bytes, batch needs %zu",
                 temp_bytes, P.total_bytes);
@@ -995,7 +998,7 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
       split_hip_batch_result& r = results[P.order[j]];
       r.iterations = back[j].iter;
       r.shift = back[j].shift;
-      r.num_segments = compress ? back[j].nseg : -1;
+      r.num_segments = compress && connectivity != 0 ? back[j].nseg : -1;
     }
   } else {
     SPLIT_HIP_CHECK(hipStreamSynchronize(stream));

@@ -619,6 +619,30 @@ __global__ void pack_kernel(const PackImg* imgs, uint16_t* out) {
   }
 }
 
+// The same area-average resample to float SoA planes (count x 3 x 256^2):
+// the coarse image on which the predicted centroids are refined.
+__global__ void coarse_kernel(const PackImg* imgs, float* out) {
+  const PackImg m = imgs[blockIdx.y];
+  const int64_t n = (int64_t)m.W * m.H;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < kSide * kSide;
+       e += gridDim.x * blockDim.x) {
+    const int Y = e / kSide, X = e - Y * kSide;
+    const int x0 = (int)((int64_t)X * m.W / kSide);
+    const int x1 = std::max((int)((int64_t)(X + 1) * m.W / kSide), x0 + 1);
+    const int y0 = (int)((int64_t)Y * m.H / kSide);
+    const int y1 = std::max((int)((int64_t)(Y + 1) * m.H / kSide), y0 + 1);
+    const float cnt = (float)((x1 - x0) * (y1 - y0));
+    float* o = out + (int64_t)blockIdx.y * 3 * kSide * kSide + e;
+    for (int c = 0; c < 3; ++c) {
+      const float* pl = m.pts + c * n;
+      float s = 0.f;
+      for (int yy = y0; yy < y1; ++yy)
+        for (int xx = x0; xx < x1; ++xx) s += pl[(int64_t)yy * m.W + xx];
+      o[(int64_t)c * kSide * kSide] = __fdiv_rn(s, cnt);
+    }
+  }
+}
+
 __global__ void scatter_centroids(const float* P, float* const* dst, int B) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= B * kOut) return;
@@ -873,6 +897,29 @@ int split_hip_ws_pack(const split_hip_image_desc* images, int32_t count,
   hipLaunchKernelGGL(pack_kernel, dim3(64, count), dim3(256), 0, stream,
                      static_cast<const PackImg*>(temp), input);
   SPLIT_HIP_CHECK_LAUNCH("ws pack");
+  SPLIT_HIP_CHECK(hipStreamSynchronize(stream));
+  return SPLIT_HIP_OK;
+}
+
+int split_hip_ws_coarse(const split_hip_image_desc* images, int32_t count,
+                        float* planes, void* temp, size_t temp_bytes,
+                        split_hip_stream_t stream) {
+  if (!images || count < 1 || !planes || !temp)
+    return fail(SPLIT_HIP_ERR_ARGUMENT, "images, planes or temp missing");
+  if (temp_bytes < (size_t)count * sizeof(PackImg))
+    return fail(SPLIT_HIP_ERR_TEMP, "temp holds %zu bytes, coarse needs %zu",
+                temp_bytes, (size_t)count * sizeof(PackImg));
+  std::vector<PackImg> h(count);
+  for (int i = 0; i < count; ++i) {
+    if (!images[i].points || images[i].width < 1 || images[i].height < 1)
+      return fail(SPLIT_HIP_ERR_ARGUMENT, "image %d: no points or empty", i);
+    h[i] = PackImg{images[i].points, images[i].width, images[i].height};
+  }
+  SPLIT_HIP_CHECK(hipMemcpyAsync(temp, h.data(), count * sizeof(PackImg),
+                                 hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(coarse_kernel, dim3(64, count), dim3(256), 0, stream,
+                     static_cast<const PackImg*>(temp), planes);
+  SPLIT_HIP_CHECK_LAUNCH("ws coarse");
   SPLIT_HIP_CHECK(hipStreamSynchronize(stream));
   return SPLIT_HIP_OK;
 }

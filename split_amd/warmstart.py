@@ -8,7 +8,14 @@ fills them by sampling pixels (samples/src/main.cu:230-233); this module
 fills them with the network's prediction instead.
 
     net = WarmStartNet.load()                       # committed weights
-    net.predict([dict(points=P, width=w, height=h, centroids=C)])  # C (3, 16)
+    imgs = [dict(points=P, width=w, height=h, centroids=C)]       # C (3, 16)
+    net.predict_refined(imgs)   # prediction, refined on the 256^2 resample
+    net.predict(imgs)           # the prediction alone
+
+The refinement runs the batched clustering (connectivity 0) on the same
+area-average resample the network reads, as float planes, from the
+prediction until shift < 1e-4: a coarse-to-fine start whose iterations cost
+1/16 of a full-size one on a 1024^2 image.
 
 Training (on the GPU, by the project's own forward/backward kernels):
 
@@ -96,6 +103,39 @@ def pack(images, out=None):
     return out
 
 
+def coarse(images, out=None):
+    """(count, 3, 256 * 256) float SoA planes of images (dicts with points,
+    width, height): the area-average resample the network input uses."""
+    torch = _torch()
+    d = _descs(images, False)
+    n = len(images)
+    if out is None:
+        out = torch.empty((n, 3, SIDE * SIDE), dtype=torch.float32, device="cuda")
+    tmp = torch.empty(max(16 * n, 256), dtype=torch.uint8, device="cuda")
+    _check(lib().split_hip_ws_coarse(d, n, out.data_ptr(), tmp.data_ptr(), tmp.numel(),
+                                     _stream()), "ws_coarse")
+    return out
+
+
+def refine_on_coarse(images, planes, max_iter: int = 100, threshold: float = 1e-4,
+                     labels=None, temp=None):
+    """Lloyd iterations on the coarse images (planes, from coarse()) from
+    each image's current centroids, written back in place: the clustering
+    alone (batched path, connectivity 0).  Returns the coarse iteration
+    counts."""
+    torch = _torch()
+    count = len(images)
+    if labels is None:
+        labels = torch.empty((count, SIDE * SIDE), dtype=torch.int32, device="cuda")
+    cimgs = [dict(points=planes[i], width=SIDE, height=SIDE, centroids=images[i]["centroids"],
+                  labels=labels[i]) for i in range(count)]
+    if temp is None:
+        temp = torch.empty(segment_batch_workspace(cimgs), dtype=torch.uint8, device="cuda")
+    res = segment_batch(cimgs, max_iter=max_iter, threshold=threshold, connectivity=0,
+                        compress=False, temp=temp)
+    return [r.iterations for r in res]
+
+
 class WarmStartNet:
     """Parameters on the device plus the kernels' workspace."""
 
@@ -156,6 +196,15 @@ class WarmStartNet:
         d = _descs(images, True)
         _check(lib().split_hip_ws_predict(self.params.data_ptr(), d, n, ws.data_ptr(),
                                           ws.numel(), _stream()), "ws_predict")
+
+    def predict_refined(self, images, max_iter: int = 100, threshold: float = 1e-4):
+        """The shipped warm start: predict(), then Lloyd iterations from the
+        prediction on each image's 256^2 coarse resample (refine_on_coarse)
+        until shift < threshold or max_iter.  Returns the coarse iteration
+        counts."""
+        self.predict(images)
+        return refine_on_coarse(images, coarse(images), max_iter=max_iter,
+                                threshold=threshold)
 
 
 def adam(params, grads, m, v, step: int, lr: float, b1=0.9, b2=0.999, eps=1e-8) -> None:
@@ -287,11 +336,15 @@ def eval_specs(count: int = 256, side: int = 1024, seed: int = 0xE7A1_0004):
 
 def evaluate(net: "WarmStartNet", count: int = 256, side: int = 1024, reps: int = 1,
              separate: bool = False, log=print) -> dict:
-    """Plain (seeded-sample init) against warm-started (network init) batched
-    segmentation of the evaluation images: Lloyd iterations to shift < 1e-4
-    (at most 100), final inertia, and time per run (init + segmentation).
-    With `separate`, also the plain method as one segment() call per image
-    (the unbatched GPU path), whose results equal the batched plain run."""
+    """Plain (seeded-sample init) against warm-started batched segmentation
+    of the evaluation images: Lloyd iterations to shift < 1e-4 (at most 100),
+    final inertia, and time per run (init + segmentation).  Warm starts:
+    'warm' = the network's prediction; 'warm_coarse' = the prediction refined
+    by Lloyd iterations on the 256^2 coarse image (coarse() +
+    refine_on_coarse(), shift < 1e-4); 'plain_coarse' (the ablation) = the
+    seeded sample refined the same way.  With `separate`, also the plain
+    method as one segment() call per image (the unbatched GPU path), whose
+    results equal the batched plain run."""
     torch = _torch()
     from . import gen_images, inertia, segment, segment_workspace
     specs = eval_specs(count, side)
@@ -306,7 +359,13 @@ def evaluate(net: "WarmStartNet", count: int = 256, side: int = 1024, reps: int 
     temp = torch.empty(segment_batch_workspace(imgs), dtype=torch.uint8, device="cuda")
     itemp = torch.empty(int(lib().split_hip_inertia_workspace()), dtype=torch.uint8,
                         device="cuda")
+    cplanes = torch.empty((count, 3, SIDE * SIDE), dtype=torch.float32, device="cuda")
+    clabels = torch.empty((count, SIDE * SIDE), dtype=torch.int32, device="cuda")
+    ctemp = torch.empty(segment_batch_workspace(
+        [dict(points=cplanes[i], width=SIDE, height=SIDE, centroids=cent[i],
+              labels=clabels[i]) for i in range(count)]), dtype=torch.uint8, device="cuda")
     net.workspace(count)
+    coarse_iters = {}
 
     def plain_init():
         for i, (_, init_seed) in enumerate(specs):
@@ -315,8 +374,17 @@ def evaluate(net: "WarmStartNet", count: int = 256, side: int = 1024, reps: int 
     def warm_init():
         net.predict(imgs)
 
+    def refined(init, name):
+        def run():
+            init()
+            coarse(imgs, out=cplanes)
+            coarse_iters[name] = refine_on_coarse(imgs, cplanes, labels=clabels, temp=ctemp)
+        return run
+
     out = {}
-    for name, init in (("plain", plain_init), ("warm", warm_init)):
+    for name, init in (("plain", plain_init), ("warm", warm_init),
+                       ("warm_coarse", refined(warm_init, "warm_coarse")),
+                       ("plain_coarse", refined(plain_init, "plain_coarse"))):
         init()
         segment_batch(imgs, max_iter=100, threshold=1e-4, connectivity=8, temp=temp)
         torch.cuda.synchronize()
@@ -333,9 +401,13 @@ def evaluate(net: "WarmStartNet", count: int = 256, side: int = 1024, reps: int 
         ine = np.array([inertia(planes[i], cent[i], temp=itemp) for i in range(count)])
         out[name] = {"seconds": best, "iterations": its, "inertia": ine,
                      "segments": np.array([r.num_segments for r in res])}
+        if name in coarse_iters:
+            out[name]["coarse_iterations"] = np.array(coarse_iters[name])
         if log:
+            extra = (f", coarse iterations mean {np.mean(coarse_iters[name]):.2f}"
+                     if name in coarse_iters else "")
             log(f"eval {name}: {best * 1e3:.1f} ms, iterations mean {its.mean():.2f} "
-                f"(median {np.median(its):.0f}, hit cap {(its >= 100).sum()}), "
+                f"(median {np.median(its):.0f}, hit cap {(its >= 100).sum()}){extra}, "
                 f"inertia sum {ine.sum():.2f}")
     if separate:
         stemp = torch.empty(segment_workspace(side, side, K), dtype=torch.uint8,
@@ -360,29 +432,42 @@ def evaluate(net: "WarmStartNet", count: int = 256, side: int = 1024, reps: int 
         out["separate"] = {"seconds": dt, "same_iterations": same}
         if log:
             log(f"eval separate plain calls: {dt * 1e3:.1f} ms (same iterations: {same})")
-    ratio = out["warm"]["inertia"] / out["plain"]["inertia"]
-    out["summary"] = {
-        "images": count, "side": side, "pixels": count * n,
-        "plain_ms": out["plain"]["seconds"] * 1e3, "warm_ms": out["warm"]["seconds"] * 1e3,
-        "plain_mpix_s": count * n / out["plain"]["seconds"] / 1e6,
-        "warm_mpix_s": count * n / out["warm"]["seconds"] / 1e6,
-        "speedup": out["plain"]["seconds"] / out["warm"]["seconds"],
-        "plain_iters_mean": float(out["plain"]["iterations"].mean()),
-        "warm_iters_mean": float(out["warm"]["iterations"].mean()),
-        "inertia_ratio_total": float(out["warm"]["inertia"].sum() /
-                                     out["plain"]["inertia"].sum()),
-        "inertia_ratio_median": float(np.median(ratio)),
-        "inertia_ratio_max": float(ratio.max()),
-        "images_warm_worse_0p5pct": int((ratio > 1.005).sum()),
-    }
+    P = out["plain"]
+    summ = {"images": count, "side": side, "pixels": count * n,
+            "plain_ms": P["seconds"] * 1e3,
+            "plain_mpix_s": count * n / P["seconds"] / 1e6,
+            "plain_iters_mean": float(P["iterations"].mean()),
+            "plain_cap_hits": int((P["iterations"] >= 100).sum())}
+    for name in ("warm", "warm_coarse", "plain_coarse"):
+        o = out[name]
+        ratio = o["inertia"] / P["inertia"]
+        summ.update({
+            f"{name}_ms": o["seconds"] * 1e3,
+            f"{name}_mpix_s": count * n / o["seconds"] / 1e6,
+            f"{name}_iters_mean": float(o["iterations"].mean()),
+            f"{name}_cap_hits": int((o["iterations"] >= 100).sum()),
+            f"{name}_inertia_ratio_total": float(o["inertia"].sum() / P["inertia"].sum()),
+            f"{name}_inertia_ratio_median": float(np.median(ratio)),
+            f"{name}_images_worse_0p5pct": int((ratio > 1.005).sum()),
+        })
+        if "coarse_iterations" in o:
+            ci = float(o["coarse_iterations"].mean())
+            summ[f"{name}_coarse_iters_mean"] = ci
+            # coarse iterations at their pixel share of a full-size one
+            summ[f"{name}_pixel_iters_mean"] = float(o["iterations"].mean()) + ci * SIDE * SIDE / n
+    # earlier names of the network-only figures
+    summ.update({"warm_mpix_s": summ["warm_mpix_s"], "speedup": P["seconds"] / out["warm"]["seconds"],
+                 "inertia_ratio_total": summ["warm_inertia_ratio_total"]})
     if separate:
         sep = out["separate"]["seconds"]
-        out["summary"].update({
+        summ.update({
             "separate_plain_ms": sep * 1e3,
             "separate_plain_mpix_s": count * n / sep / 1e6,
             "speedup_batched_warm_vs_separate_plain": sep / out["warm"]["seconds"],
+            "speedup_batched_warm_coarse_vs_separate_plain": sep / out["warm_coarse"]["seconds"],
             "separate_same_iterations": out["separate"]["same_iterations"],
         })
+    out["summary"] = summ
     return out
 
 
