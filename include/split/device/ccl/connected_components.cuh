@@ -13,8 +13,10 @@ namespace ccl
 /// row-major; do_labels[p] = the largest linear pixel index of p's component
 /// (the fixed point the reference's max-propagation converges to).  The
 /// union-find always completes, so i_max_iterations is accepted and unused.
-/// i_connectivity: 4 (the reference's) or 8.  Uses the first 256 bytes of
-/// dio_temp (a cluster_workspace block is far larger).
+/// i_connectivity: 4 (the reference's) or 8.  Uses the first
+/// split_hip_connected_components_workspace(W, H) bytes of dio_temp, about
+/// 0.375 bytes per pixel (a cluster_workspace block, which the reference's
+/// callers pass, is larger).
 /// Throws split::device_error on failure.
 SPLIT_API void connected_components(const_view2d<int> di_labels,
                                     device_ptr<void> dio_temp,

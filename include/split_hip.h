@@ -175,11 +175,12 @@ int split_hip_uniform_random_initialize(const float* points, int64_t n,
 
 /* ---- connected components -------------------------------------------- */
 
-/* Temp needed by split_hip_connected_components: 256 bytes (the failure
- * flag of the check pass).  The reference ignores its temp argument and
- * allocates 2N+3R ints internally
+/* Temp needed by split_hip_connected_components: a linked-root bitmap (one
+ * bit per pixel) and the labels and local roots of every 64x64 tile's left
+ * and right columns, about 0.375 bytes per pixel plus 256.  The reference
+ * ignores its temp argument and allocates 2N+3R ints internally
  * (split/src/split/device/ccl/connected_components.cu:93-94,140-142); its
- * callers pass the clustering temp, which is far larger. */
+ * callers pass the clustering temp (at least N ints), which is larger. */
 size_t split_hip_connected_components_workspace(int32_t width, int32_t height);
 
 /* Replaces split::device::ccl::connected_components
@@ -189,9 +190,8 @@ size_t split_hip_connected_components_workspace(int32_t width, int32_t height);
  * linear pixel index of p's connected component of equal labels — the value
  * the reference's max-propagation reaches at convergence — under 4- or
  * 8-connectivity.  The union-find always runs to completion, so max_iter is
- * accepted for signature parity and ignored.  A final pass checks that every
- * pixel carries a root and that equal-label neighbours agree, repairing (one
- * union + a second flatten) what does not. */
+ * accepted for signature parity and ignored.  Passes: per-tile LDS
+ * union-find, tile-seam unions, root jumps, a per-tile final resolution. */
 int split_hip_connected_components(const int32_t* labels, int32_t width,
                                    int32_t height, void* temp,
                                    size_t temp_bytes, int32_t* out,
