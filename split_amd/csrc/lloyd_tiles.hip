@@ -416,8 +416,12 @@ struct BoundedWalker {
 
 
 // Launch s of a tiled run (FULL: s == 1, every pixel assigned).
+// k <= 64: 1024-thread blocks at 4 waves per SIMD (<= 128 VGPRs).  Larger
+// k: 256-thread blocks at 2 waves per SIMD, whose 2 or 4 centroids per lane
+// need up to 186 VGPRs (capped at 128 they spilled 29 / 59 VGPRs to scratch;
+// on the 8192^2 k = 256 image 1,756 -> 1,723 us per iteration without).
 template <int CPL, int MODE, bool FULL>
-__global__ __launch_bounds__(1024) void tiles_step(TileArgs a, int s) {
+__global__ __launch_bounds__(CPL >= 2 ? 256 : 1024, CPL >= 2 ? 2 : 1) void tiles_step(TileArgs a, int s) {
   extern __shared__ long long lds64[];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
@@ -655,9 +659,9 @@ __global__ void tiles_finish(const LloydState* st, const float* cbuf,
 // One block per CU where the LDS allows (k <= 128: up to 129 KB): the
 // waves of a CU then share one pool of tiles in the bounded iterations and
 // finish together.
-static int tiles_threads(int k) { return k <= 128 ? 1024 : 256; }
+static int tiles_threads(int k) { return k <= 64 ? 1024 : 256; }
 // Step launches use at most this many blocks.
-static int tiles_max_blocks(int k) { return k <= 128 ? 512 : 1024; }
+static int tiles_max_blocks(int k) { return k <= 64 ? 512 : 1024; }
 // Units (pairs of adjacent tiles) of n pixels in rows of width.
 static int64_t tile_units(int64_t n, int width) {
   if (width <= 0) return 0;
