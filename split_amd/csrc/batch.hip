@@ -22,10 +22,11 @@
 // table, stop flag.  Images finish independently; later launches skip
 // their tiles.
 //
-// After the iterations: the labels (from the per-pixel state), connected
-// components (32x32 tiles, all images in one grid), the root bitmap, a
-// per-image scan and the rank map — the same passes as the single-image
-// path.
+// After the iterations: the labels (from the per-pixel state) and
+// connected components with the single-image passes (ccl_device.hpp) over
+// every image's 64x64 tiles in one grid: tile pass, seam merge, root jumps,
+// a per-image scan of the root bitmap, and the final per-tile resolution
+// that writes each pixel's root or dense rank.
 //
 // The whole sequence is captured once into a hipGraph per (temp, batch
 // shape, settings) and replayed; per-image buffer pointers live in a
@@ -57,14 +58,16 @@ struct BatchImg {
   const float* centroids_in;
   float* centroids_out;
   int W, H, k, tiles_x;  // 16x16 Lloyd tiles
-  int ctiles_x, pad0;    // 32x32 CC tiles
+  int ctiles_x, pad0;    // 64x64 CC tiles
   int64_t tile0;         // first global Lloyd tile
   int64_t ctile0;        // first global CC tile
   int64_t px0;           // first global pixel (multiple of 64)
   float* cbuf;           // 2 x 3k
   long long* totals;     // 4k
-  unsigned* bitmap;      // 2*ceil(n/64) words
+  unsigned* bitmap;      // 2*ceil(n/64) words: local roots, then roots
+  unsigned* linked;      // 2*ceil(n/64) words: roots linked under another
   unsigned* prefix;      // ceil(n/32) words
+  int32_t* edge;         // 4 x (CC tiles x 64): the tiles' edge columns
   uint32_t* pstate;      // per-pixel Lloyd state (lloyd_bounded.hpp), n
   float* fcum;           // per-centroid bound consumption, k
   // device state
@@ -77,7 +80,6 @@ struct BatchImg {
 
 struct BatchArgs {
   BatchImg* img;
-  unsigned* ccl_flag;  // connected-component check failures
   int count;
   int64_t tiles;   // total Lloyd tiles
   int64_t ctiles;  // total CC tiles
@@ -124,6 +126,7 @@ struct PxImg {
   int32_t* labels;
   int32_t* segments;
   unsigned* bitmap;
+  unsigned* linked;
   unsigned* prefix;
   const uint32_t* pstate;
   int64_t n, px0;
@@ -131,7 +134,7 @@ struct PxImg {
 };
 
 __device__ __forceinline__ PxImg px_image(const BatchImg& m) {
-  return PxImg{m.labels, m.segments, m.bitmap, m.prefix, m.pstate,
+  return PxImg{m.labels, m.segments, m.bitmap, m.linked, m.prefix, m.pstate,
                (int64_t)m.W * m.H, m.px0, m.W, m.H, m.iter, m.bounded};
 }
 
@@ -547,69 +550,119 @@ __global__ void batch_finish(BatchArgs a) {
   for (int i = threadIdx.x; i < 3 * m.k; i += blockDim.x) m.centroids_out[i] = src[i];
 }
 
-template <int CONN>
-__global__ __launch_bounds__(kCcThreads) void batch_ccl_tile(BatchArgs a) {
-  __shared__ int lab[kCcTilePx];
-  __shared__ int par[kCcTilePx];
-  if (blockIdx.x == 0 && threadIdx.x == 0) *a.ccl_flag = 0u;
-  for (int64_t c = blockIdx.x; c < a.ctiles; c += gridDim.x) {
-    const int i = find_image<&BatchImg::ctile0>(a.img, a.count, c);
+// The batched connected components: the single-image passes
+// (ccl_device.hpp) over every image's 64x64 tiles and bitmap words.
+constexpr int kCcSide = 64;
+using BatchCcTile = CcTile<kCcSide>;
+
+__device__ __forceinline__ CcArgs image_cc_args(const BatchImg& m,
+                                                bool compress) {
+  const int64_t nt = (int64_t)m.ctiles_x * ((m.H + kCcSide - 1) / kCcSide);
+  CcArgs c{};
+  c.labels = m.labels;
+  c.out = m.segments;
+  c.W = m.W;
+  c.H = m.H;
+  c.ntx = m.ctiles_x;
+  c.linked = m.linked;
+  c.local_root = compress ? m.bitmap : nullptr;
+  c.llab = m.edge;
+  c.rlab = m.edge + nt * kCcSide;
+  c.lroot = m.edge + 2 * nt * kCcSide;
+  c.rroot = m.edge + 3 * nt * kCcSide;
+  return c;
+}
+
+// Calls f(m, tx, ty) for the block's contiguous share of the global CC
+// tiles: one binary search for its first tile's image, then a walk forward
+// (a search per tile cost ~8 dependent loads of latency each).
+template <class F>
+__device__ __forceinline__ void for_each_cc_tile(const BatchArgs& a, F&& f) {
+  const int64_t c0 = a.ctiles * blockIdx.x / gridDim.x;
+  const int64_t c1 = a.ctiles * (blockIdx.x + 1) / gridDim.x;
+  if (c0 >= c1) return;
+  int i = find_image<&BatchImg::ctile0>(a.img, a.count, c0);
+  int64_t next = i + 1 < a.count ? a.img[i + 1].ctile0 : INT64_MAX;
+  for (int64_t c = c0; c < c1; ++c) {
+    while (c >= next) {
+      ++i;
+      next = i + 1 < a.count ? a.img[i + 1].ctile0 : INT64_MAX;
+    }
     const BatchImg& m = a.img[i];
     const int64_t local = c - m.ctile0;
-    const int ty = (int)(local / m.ctiles_x), tx = (int)(local % m.ctiles_x);
-    ccl_tile_body<CONN>(m.labels, m.W, m.H, m.segments, tx, ty, lab, par);
-    __syncthreads();
+    const int ty = (int)(local / m.ctiles_x);
+    const int tx = (int)(local - (int64_t)ty * m.ctiles_x);
+    f(m, tx, ty);
+  }
+}
+
+// Zeroes every image's linked-root and local-root bitmaps (images strided
+// over blockIdx.y; 8-byte stores: the allocations are 256-byte aligned and
+// hold 2 ceil(n/64) words).
+__global__ __launch_bounds__(256) void batch_cc_clear(BatchArgs a) {
+  for (int i = blockIdx.y; i < a.count; i += gridDim.y) {
+    const BatchImg& m = a.img[i];
+    const int64_t pairs = ((int64_t)m.W * m.H + 63) / 64;  // uint2 per bitmap
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+         q < pairs; q += (int64_t)gridDim.x * blockDim.x) {
+      reinterpret_cast<uint2*>(m.linked)[q] = make_uint2(0u, 0u);
+      reinterpret_cast<uint2*>(m.bitmap)[q] = make_uint2(0u, 0u);
+    }
   }
 }
 
 template <int CONN>
-__global__ __launch_bounds__(3 * kCcTile) void batch_ccl_merge(BatchArgs a) {
-  for (int64_t c = blockIdx.x; c < a.ctiles; c += gridDim.x) {
-    const int i = find_image<&BatchImg::ctile0>(a.img, a.count, c);
-    const BatchImg& m = a.img[i];
-    const int64_t local = c - m.ctile0;
-    const int ty = (int)(local / m.ctiles_x), tx = (int)(local % m.ctiles_x);
-    ccl_merge_body<CONN>(m.labels, m.W, m.H, m.segments, tx, ty, threadIdx.x);
+__global__ __launch_bounds__(BatchCcTile::kThreads) void batch_cc_tile(
+    BatchArgs a, int compress) {
+  __shared__ int lab[BatchCcTile::kPx];
+  __shared__ int par[BatchCcTile::kPx];
+  for_each_cc_tile(a, [&](const BatchImg& m, int tx, int ty) {
+    cc_tile_body<CONN, kCcSide>(image_cc_args(m, compress != 0), tx, ty, lab,
+                                par);
+    __syncthreads();  // lab / par are reused by the next tile
+  });
+}
+
+template <int CONN>
+__global__ __launch_bounds__(3 * kCcSide) void batch_cc_merge(BatchArgs a) {
+  for_each_cc_tile(a, [&](const BatchImg& m, int tx, int ty) {
+    cc_merge_body<CONN, kCcSide>(image_cc_args(m, false), tx, ty);
+  });
+}
+
+// Root jumps over every image's linked-root words, in decreasing order of
+// the padded global pixel space (each image's ancestors first): wave w of
+// W takes the w-th share from the top and walks it downwards.
+__global__ __launch_bounds__(256) void batch_cc_jump(BatchArgs a) {
+  const int64_t units = a.px >> 6;
+  const int wpb = (int)(blockDim.x >> 6);
+  const int64_t nw = (int64_t)gridDim.x * wpb;
+  const int64_t gw = nw - 1 - ((int64_t)blockIdx.x * wpb +
+                               __builtin_amdgcn_readfirstlane(
+                                   (int)(threadIdx.x >> 6)));
+  const int64_t u0 = units * gw / nw, u1 = units * (gw + 1) / nw;
+  if (u0 >= u1) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  int i = find_image<&BatchImg::px0>(a.img, a.count, (u1 - 1) << 6);
+  // two lanes per 64-pixel unit (one bitmap word each), 32 units per pass;
+  // i: the image of the pass's top unit
+  for (int64_t ub = u1 - 1; ub >= u0; ub -= 32) {
+    const int64_t u = ub - (lane >> 1);
+    const bool ok = u >= u0;
+    const int64_t g0 = (ok ? u : u0) << 6;
+    int j = i;
+    while (j > 0 && a.img[j].px0 > g0) --j;
+    if (ok) {
+      const BatchImg& m = a.img[j];
+      const int64_t w = ((g0 - m.px0) >> 5) + 1 - (lane & 1);
+      if (w * 32 < (int64_t)m.W * m.H) cc_jump_word(m.segments, m.linked, w);
+    }
+    i = __builtin_amdgcn_readlane(j, kWave - 1);  // the pass's lowest unit
   }
 }
 
-// Root jumps (ccl_jump_one) over the padded global pixel space.
-__global__ __launch_bounds__(256) void batch_ccl_jump(BatchArgs a) {
-  const int lane = threadIdx.x & (kWave - 1);
-  for_each_unit(a, [&](const PxImg& m, int64_t q0) {
-    const int64_t q = q0 + lane;
-    if (q < m.n) ccl_jump_one(m.segments, q, m.W);
-  });
-}
-
-// Check of the flattened forests (ccl_check_one) over the padded global
-// pixel space.
-template <int CONN>
-__global__ __launch_bounds__(256) void batch_ccl_check(BatchArgs a) {
-  const int lane = threadIdx.x & (kWave - 1);
-  for_each_unit(a, [&](const PxImg& m, int64_t q0) {
-    const int64_t q = q0 + lane;
-    if (q < m.n) ccl_check_one<CONN>(m.labels, m.W, m.H, m.segments, q, a.ccl_flag);
-  });
-}
-
-// Flatten + root bitmap over the padded global pixel space.  recheck: run
-// only when the check pass found a failure.
-__global__ __launch_bounds__(256) void batch_flatten(BatchArgs a, int compress,
-                                                     int recheck) {
-  if (recheck && *a.ccl_flag == 0u) return;
-  const int lane = threadIdx.x & (kWave - 1);
-  for_each_unit(a, [&](const PxImg& m, int64_t q0) {
-    const int64_t q = q0 + lane;
-    const bool root = q < m.n && ccl_flatten_one(m.segments, q) == q;
-    const uint64_t bits = ballot(root);
-    if (compress && lane == 0 && q0 < m.n)
-      *reinterpret_cast<uint2*>(m.bitmap + (q0 >> 5)) =
-          make_uint2((uint32_t)bits, (uint32_t)(bits >> 32));
-  });
-}
-
-// Exclusive popcount prefix of each image's root bitmap, one block per image.
+// Per image (one block): roots = local roots not linked, written back into
+// the bitmap, and the exclusive popcount prefix of its words.
 __global__ __launch_bounds__(1024) void batch_scan(BatchArgs a) {
   BatchImg& m = a.img[blockIdx.x];
   const int64_t n = (int64_t)m.W * m.H;
@@ -618,7 +671,11 @@ __global__ __launch_bounds__(1024) void batch_scan(BatchArgs a) {
   const int64_t per = (words + nt - 1) / nt;
   const int64_t w0 = (int64_t)threadIdx.x * per;
   unsigned sum = 0;
-  for (int64_t w = w0; w < w0 + per && w < words; ++w) sum += __popc(m.bitmap[w]);
+  for (int64_t w = w0; w < w0 + per && w < words; ++w) {
+    const unsigned r = m.bitmap[w] & ~m.linked[w];
+    m.bitmap[w] = r;
+    sum += __popc(r);
+  }
   __shared__ unsigned s[1024];
   s[threadIdx.x] = sum;
   __syncthreads();
@@ -636,16 +693,15 @@ __global__ __launch_bounds__(1024) void batch_scan(BatchArgs a) {
   if (threadIdx.x == nt - 1) m.nseg = (int)s[nt - 1];
 }
 
-__global__ __launch_bounds__(256) void batch_map(BatchArgs a) {
-  const int lane = threadIdx.x & (kWave - 1);
-  for_each_unit(a, [&](const PxImg& m, int64_t q0) {
-    const int64_t q = q0 + lane;
-    if (q < m.n) {
-      const uint32_t v = (uint32_t)m.segments[q];
-      const uint32_t wd = v >> 5;
-      m.segments[q] =
-          (int32_t)(m.prefix[wd] + __popc(m.bitmap[wd] & ((1u << (v & 31)) - 1u)));
-    }
+// Every pixel's root (or, with compress, its dense rank), per tile.
+__global__ __launch_bounds__(BatchCcTile::kThreads) void batch_cc_final(
+    BatchArgs a, int compress) {
+  __shared__ int sv[BatchCcTile::kPx];
+  for_each_cc_tile(a, [&](const BatchImg& m, int tx, int ty) {
+    cc_final_body<kCcSide>(image_cc_args(m, false), tx, ty,
+                           compress ? m.bitmap : nullptr, m.prefix, nullptr,
+                           sv);
+    __syncthreads();  // sv is reused by the next tile
   });
 }
 
@@ -684,7 +740,7 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
 
   P->img.assign(count, BatchImg{});
   char* t = static_cast<char*>(temp);
-  size_t o = align_up((size_t)count * sizeof(BatchImg), 256) + 256;  // + flag
+  size_t o = align_up((size_t)count * sizeof(BatchImg), 256);
   int64_t tiles = 0, ctiles = 0, px = 0;
   for (int j = 0; j < count; ++j) {
     BatchImg& m = P->img[j];
@@ -701,12 +757,14 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
     m.k = e.k;
     m.bounded = j >= P->nplain;
     m.tiles_x = (e.width + kTileSide - 1) / kTileSide;
-    m.ctiles_x = (e.width + kCcTile - 1) / kCcTile;
+    m.ctiles_x = (e.width + kCcSide - 1) / kCcSide;
     m.tile0 = tiles;
     m.ctile0 = ctiles;
     m.px0 = px;
     tiles += (int64_t)m.tiles_x * ((e.height + kTileSide - 1) / kTileSide);
-    ctiles += (int64_t)m.ctiles_x * ((e.height + kCcTile - 1) / kCcTile);
+    const int64_t img_ctiles =
+        (int64_t)m.ctiles_x * ((e.height + kCcSide - 1) / kCcSide);
+    ctiles += img_ctiles;
     px += (n + 63) / 64 * 64;
     auto take = [&](size_t bytes) {
       char* p = t ? t + o : nullptr;
@@ -716,6 +774,8 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
     m.cbuf = reinterpret_cast<float*>(take(6 * (size_t)e.k * sizeof(float)));
     m.totals = reinterpret_cast<long long*>(take(4 * (size_t)e.k * 8));
     m.bitmap = reinterpret_cast<unsigned*>(take(2 * (size_t)((n + 63) / 64) * 4));
+    m.linked = reinterpret_cast<unsigned*>(take(2 * (size_t)((n + 63) / 64) * 4));
+    m.edge = reinterpret_cast<int32_t*>(take(4 * (size_t)img_ctiles * kCcSide * 4));
     m.prefix = reinterpret_cast<unsigned*>(take((size_t)((n + 31) / 32) * 4));
     m.pstate = reinterpret_cast<uint32_t*>(take(((size_t)n + kStatePad) * 4));
     m.fcum = reinterpret_cast<float*>(take((size_t)e.k * sizeof(float)));
@@ -854,33 +914,25 @@ static void enqueue_batch(const BatchArgs& a, const Groups& G, int conn,
                        a);
   hipLaunchKernelGGL(batch_finish, dim3(a.count), dim3(256), 0, stream, a);
   if (conn == 0) return;  // clustering only
-  const int cgrid = (int)std::min<int64_t>(a.ctiles, (int64_t)cus * 16);
+  const int cgrid = (int)std::min<int64_t>(a.ctiles, (int64_t)cus * 8);
+  hipLaunchKernelGGL(batch_cc_clear, dim3(16, std::min(a.count, 65535)),
+                     dim3(256), 0, stream, a);
   if (conn == 8) {
-    hipLaunchKernelGGL(batch_ccl_tile<8>, dim3(cgrid), dim3(kCcThreads), 0,
-                       stream, a);
-    hipLaunchKernelGGL(batch_ccl_merge<8>, dim3(cgrid), dim3(3 * kCcTile), 0,
+    hipLaunchKernelGGL(batch_cc_tile<8>, dim3(cgrid),
+                       dim3(BatchCcTile::kThreads), 0, stream, a, compress);
+    hipLaunchKernelGGL(batch_cc_merge<8>, dim3(cgrid), dim3(3 * kCcSide), 0,
                        stream, a);
   } else {
-    hipLaunchKernelGGL(batch_ccl_tile<4>, dim3(cgrid), dim3(kCcThreads), 0,
-                       stream, a);
-    hipLaunchKernelGGL(batch_ccl_merge<4>, dim3(cgrid), dim3(3 * kCcTile), 0,
+    hipLaunchKernelGGL(batch_cc_tile<4>, dim3(cgrid),
+                       dim3(BatchCcTile::kThreads), 0, stream, a, compress);
+    hipLaunchKernelGGL(batch_cc_merge<4>, dim3(cgrid), dim3(3 * kCcSide), 0,
                        stream, a);
   }
-  hipLaunchKernelGGL(batch_ccl_jump, dim3(8 * cus), dim3(256), 0, stream, a);
-  hipLaunchKernelGGL(batch_flatten, dim3(8 * cus), dim3(256), 0, stream, a,
-                     compress, 0);
-  if (conn == 8)
-    hipLaunchKernelGGL(batch_ccl_check<8>, dim3(8 * cus), dim3(256), 0, stream,
-                       a);
-  else
-    hipLaunchKernelGGL(batch_ccl_check<4>, dim3(8 * cus), dim3(256), 0, stream,
-                       a);
-  hipLaunchKernelGGL(batch_flatten, dim3(8 * cus), dim3(256), 0, stream, a,
-                     compress, 1);
-  if (compress) {
+  hipLaunchKernelGGL(batch_cc_jump, dim3(32 * cus), dim3(256), 0, stream, a);
+  if (compress)
     hipLaunchKernelGGL(batch_scan, dim3(a.count), dim3(1024), 0, stream, a);
-    hipLaunchKernelGGL(batch_map, dim3(8 * cus), dim3(256), 0, stream, a);
-  }
+  hipLaunchKernelGGL(batch_cc_final, dim3(cgrid), dim3(BatchCcTile::kThreads),
+                     0, stream, a, compress);
 }
 
 struct GraphEntry {
@@ -931,8 +983,6 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
                 temp_bytes, P.total_bytes);
   BatchArgs a;
   a.img = static_cast<BatchImg*>(temp);
-  a.ccl_flag = reinterpret_cast<unsigned*>(
-      static_cast<char*>(temp) + align_up((size_t)count * sizeof(BatchImg), 256));
   a.count = count;
   a.tiles = P.tiles;
   a.ctiles = P.ctiles;

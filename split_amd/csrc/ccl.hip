@@ -45,261 +45,38 @@
 
 namespace splithip {
 
-// Tile side S of the single-image CC (kCcSide).  Thread t
-// of a block handles tile pixels t + (S*S/4) j, j = 0..3: with S = 64 a
-// wave holds one tile row per j, with S = 32 two (lanes 0-31, 32-63).
-template <int S>
-struct CcTile {
-  static constexpr int kPx = S * S;
-  static constexpr int kThreads = kPx / 4;
-  static constexpr int kRowsPerPass = kThreads / S;
-};
-
-// Inputs, outputs and scratch of one connected-components run.
-struct CcArgs {
-  const int32_t* labels;
-  int32_t* out;   // the forest, then the result
-  int W, H, ntx;
-  unsigned* linked;      // bit p: root p was linked under another (N bits)
-  unsigned* local_root;  // bit p: p is its tile's local root (compression)
-  // per tile t, rows 0..S-1: labels and local roots of the left (x0) and
-  // right (x0 + S - 1) columns
-  int32_t* llab;
-  int32_t* rlab;
-  int32_t* lroot;
-  int32_t* rroot;
-};
-
-// ORs `bits` (the first `count` <= 64 bits used) into the bitmap at bit
-// offset p0 (any alignment); lanes k0, k0 + 1, k0 + 2 write one word each.
-__device__ __forceinline__ void or_bits(unsigned* bm, int64_t p0,
-                                        uint64_t bits, int lane, int k0) {
-  const int k = lane - k0;
-  if (bits == 0 || k < 0 || k > 2) return;
-  const int s = (int)(p0 & 31);
-  // word w: bits 0..31-s, word w+1: the next 32, word w+2: the last s
-  const unsigned v = k == 0   ? (unsigned)(bits << s)
-                     : k == 1 ? (unsigned)(bits >> (32 - s))
-                              : (unsigned)((bits >> (32 - s)) >> 32);
-  if (v) atomicOr(bm + (p0 >> 5) + k, v);
-}
-
 template <int CONN, int S>
 __global__ __launch_bounds__(CcTile<S>::kThreads) void cc_tile(CcArgs a) {
-  using G = CcTile<S>;
-  __shared__ int lab[G::kPx];
-  __shared__ int par[G::kPx];
-  const int tx = blockIdx.x, ty = blockIdx.y;
-  const int x0 = tx * S, y0 = ty * S;
-  const int tw = min(S, a.W - x0), th = min(S, a.H - y0);
-  const int lane = threadIdx.x & (kWave - 1);
-  const int col = threadIdx.x & (S - 1);
-  const int rowbase = lane - col;  // first lane of this lane's row
-  const int64_t T = (int64_t)ty * a.ntx + tx;
-  int l[4], end[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int r = threadIdx.x / S + G::kRowsPerPass * j;
-    const bool in = col < tw && r < th;
-    l[j] = in ? a.labels[(int64_t)(y0 + r) * a.W + x0 + col] : INT_MIN;
-    lab[r * S + col] = l[j];
-  }
-  // each pixel's parent: the last pixel of its row run
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int r = threadIdx.x / S + G::kRowsPerPass * j;
-    const bool in = col < tw && r < th;
-    const int right = __shfl_down(l[j], 1);
-    const uint64_t m = ballot(in && col + 1 < tw && right == l[j]);
-    // bit S-1 of the row's bits is clear, so the shifted complement is
-    // nonzero below it
-    end[j] = col + (int)__builtin_ctzll(~(m >> lane));
-    par[r * S + col] = r * S + (in ? end[j] : col);
-  }
-  __syncthreads();
-  // unions between runs of consecutive rows, once per run overlap
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int r = threadIdx.x / S + G::kRowsPerPass * j;
-    const int left = __shfl_up(l[j], 1);
-    if (r == 0 || r >= th || col >= tw) continue;
-    const int li = r * S + col, c = l[j], up = li - S;
-    const bool left_same = col > 0 && left == c;
-    const int lu = lab[up];
-    const int lul = col > 0 ? lab[up - 1] : INT_MIN;
-    const bool up_same = lu == c;
-    if (up_same && !(left_same && lul == c)) lds_union(par, li, up);
-    if (CONN == 8) {
-      if (col > 0 && !left_same && !up_same && lul == c)
-        lds_union(par, li, up - 1);
-      if (col + 1 < tw && !up_same && lab[up + 1] == c)
-        lds_union(par, li, up + 1);
-    }
-  }
-  __syncthreads();
-  // roots: one find per run (at its end), shared along the run
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int r = threadIdx.x / S + G::kRowsPerPass * j;
-    const bool in = col < tw && r < th;
-    const int li = r * S + col;
-    const int found = (in && col == end[j]) ? lds_find(par, li) : 0;
-    const int rl = __shfl(found, rowbase + (in ? end[j] : col));
-    const int g = (y0 + rl / S) * a.W + x0 + (rl & (S - 1));
-    const int64_t p = (int64_t)(y0 + r) * a.W + x0 + col;
-    if (in) g_store(a.out, p, g);
-    if (a.local_root) {
-      const uint64_t roots = ballot(in && rl == li);
-      if (r < th)
-        or_bits(a.local_root, (int64_t)(y0 + r) * a.W + x0,
-                S == 64 ? roots : (roots >> rowbase) & 0xFFFFFFFFull, lane,
-                rowbase);
-    }
-    if (in && col == 0) {
-      a.llab[T * S + r] = l[j];
-      a.lroot[T * S + r] = g;
-    }
-    if (in && col == S - 1) {
-      a.rlab[T * S + r] = l[j];
-      a.rroot[T * S + r] = g;
-    }
-  }
+  __shared__ int lab[CcTile<S>::kPx];
+  __shared__ int par[CcTile<S>::kPx];
+  cc_tile_body<CONN, S>(a, blockIdx.x, blockIdx.y, lab, par);
 }
 
-// One block of three groups of S threads per tile: group 0 the top row (up,
-// up-left, up-right neighbours in the rows above), group 1 the left column
-// (left and up-left neighbours in the left tile, from the edge arrays),
-// group 2 the right column's up-right neighbour in the right tile
-// (8-connectivity).  A union is skipped where the neighbouring seam pixel
-// already joins the same two runs, which leaves about one union per pair of
-// touching runs.
 template <int CONN, int S>
 __global__ __launch_bounds__(3 * S) void cc_merge(CcArgs a) {
-  const int tx = blockIdx.x, ty = blockIdx.y;
-  const int x0 = tx * S, y0 = ty * S;
-  const int tw = min(S, a.W - x0), th = min(S, a.H - y0);
-  const int t = threadIdx.x % S;
-  const int grp = threadIdx.x / S;
-  const int64_t T = (int64_t)ty * a.ntx + tx;
-  const int W = a.W;
-  if (grp == 0) {
-    if (y0 == 0 || t >= tw) return;
-    const int x = x0 + t;
-    const int p = y0 * W + x, u = p - W;
-    const int c = a.labels[p];
-    const int lu = a.labels[u];
-    const bool left_same = t > 0 && a.labels[p - 1] == c;
-    const int lul = x > 0 ? a.labels[u - 1] : INT_MIN;
-    if (lu == c && !(left_same && lul == c)) g_union(a.out, p, u, a.linked);
-    if (CONN == 8) {
-      if (x > 0 && lu != c && !left_same && lul == c)
-        g_union(a.out, p, u - 1, a.linked);
-      if (x + 1 < W && lu != c && a.labels[u + 1] == c)
-        g_union(a.out, p, u + 1, a.linked);
-    }
-  } else if (grp == 1) {
-    if (x0 == 0 || t >= th) return;
-    const int64_t e = T * S + t, el = (T - 1) * S + t;
-    const int c = a.llab[e];
-    const int lq = a.rlab[el];
-    const bool up_in = t > 0;
-    const bool up_same = up_in && a.llab[e - 1] == c;
-    const int ul = up_in ? a.rlab[el - 1] : INT_MIN;
-    if (lq == c && !(up_same && ul == c))
-      g_union(a.out, a.lroot[e], a.rroot[el], a.linked);
-    if (CONN == 8 && up_in && lq != c && !up_same && ul == c)
-      g_union(a.out, a.lroot[e], a.rroot[el - 1], a.linked);
-  } else if (CONN == 8) {
-    if (tw < S || x0 + S >= W || t == 0 || t >= th) return;
-    const int64_t e = T * S + t, er = (T + 1) * S + t;
-    const int c = a.rlab[e];
-    if (a.llab[er - 1] == c && a.rlab[e - 1] != c && a.llab[er] != c)
-      g_union(a.out, a.rroot[e], a.lroot[er - 1], a.linked);
-  }
+  cc_merge_body<CONN, S>(a, blockIdx.x, blockIdx.y);
 }
 
-// Every root that a union linked under another points at its tree's root,
-// by pointer jumping: its word moves to its parent's current word and is
-// published after every step, so the roots below it advance through it by
-// the distance it has covered.  Words are taken in decreasing order, so
-// ancestors (larger indices) start first and descendants mostly find them
-// done.  Only a root's own thread writes its word; the stores are
-// atomicMax for their visibility to the other XCDs' loads within the pass
-// (with plain stores each XCD's progress stays in its own L2 until the
-// kernel ends).  On the 8192^2 spiral image (808k linked roots, chains of
-// up to ~7,000 roots): 653 us; a path-halving walk that writes the root at
-// its end took 3,457 us in increasing order and 1,111 us in decreasing
-// order.  On the 4096^2 k = 32 segment labels: 20 us (walk: 28 us).
+// Words in decreasing order.  Only a root's own thread writes its word; on
+// the 8192^2 spiral image (808k linked roots, chains of up to ~7,000
+// roots): 653 us; a path-halving walk that writes the root at its end took
+// 3,457 us in increasing order and 1,111 us in decreasing order, pointer
+// jumping in increasing order 5,270 us.  On the 4096^2 k = 32 segment
+// labels: 20 us (walk: 28 us).
 __global__ __launch_bounds__(256) void cc_jump(int32_t* out,
                                                const unsigned* linked,
                                                int64_t words) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t w = words - 1 - i;
-    unsigned m = linked[w];
-    while (m) {
-      const int b = 31 - __builtin_clz(m);
-      m &= ~(1u << b);
-      const int a = (int)(w * 32 + b);
-      int x = g_load(out, a);
-      for (;;) {
-        const int px = g_load(out, x);
-        if (px == x) break;
-        x = px;
-        g_raise(out, a, x);
-      }
-    }
-  }
+       i += (int64_t)gridDim.x * blockDim.x)
+    cc_jump_word(out, linked, words - 1 - i);
 }
 
-// Bitmap lookup of a root's dense rank: chunk sum + word prefix + the lower
-// bits of its word (see bitmap_scan).
-__device__ __forceinline__ int32_t root_rank(int root, const unsigned* rb,
-                                             const unsigned* prefix,
-                                             const unsigned* csum);
-
-// Per tile: every pixel's root, from the tile's forest words staged in LDS.
-// After the jump pass a pixel's word holds itself (an unlinked local root:
-// a root), the root (a linked local root, jumped), or its local root in the
-// same tile (any other pixel; the tile pass wrote it and nothing rewrote it),
-// whose own word is then the root.  So the root is the word of the pixel's
-// word when that lies in the tile, else the pixel's word.  Writes the root
-// index (rb == null) or its dense rank among the roots.
 template <int S>
 __global__ __launch_bounds__(CcTile<S>::kThreads) void cc_final(
     CcArgs a, const unsigned* rb, const unsigned* prefix,
     const unsigned* csum) {
-  using G = CcTile<S>;
-  __shared__ int sv[G::kPx];
-  const int tx = blockIdx.x, ty = blockIdx.y;
-  const int x0 = tx * S, y0 = ty * S;
-  const int tw = min(S, a.W - x0), th = min(S, a.H - y0);
-  const int col = threadIdx.x & (S - 1);
-  const double inv_w = 1.0 / (double)a.W;
-  int v[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int r = threadIdx.x / S + G::kRowsPerPass * j;
-    const bool in = col < tw && r < th;
-    v[j] = in ? g_load(a.out, (int64_t)(y0 + r) * a.W + x0 + col) : 0;
-    sv[r * S + col] = v[j];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int r = threadIdx.x / S + G::kRowsPerPass * j;
-    if (col >= tw || r >= th) continue;
-    const int64_t p = (int64_t)(y0 + r) * a.W + x0 + col;
-    const int v1 = v[j];
-    // row and column of v1 (double division, corrected: exact for v1 < 2^31)
-    int vy = (int)((double)v1 * inv_w);
-    if ((int64_t)vy * a.W > v1) --vy;
-    else if ((int64_t)(vy + 1) * a.W <= v1) ++vy;
-    const int vx = v1 - vy * a.W;
-    const bool in_tile = vy >= y0 && vy < y0 + th && vx >= x0 && vx < x0 + tw;
-    const int root = in_tile ? sv[(vy - y0) * S + (vx - x0)] : v1;
-    g_store(a.out, p, rb ? root_rank(root, rb, prefix, csum) : root);
-  }
+  __shared__ int sv[CcTile<S>::kPx];
+  cc_final_body<S>(a, blockIdx.x, blockIdx.y, rb, prefix, csum, sv);
 }
 
 // ------------------------------------------------------------ compress ---
@@ -315,7 +92,7 @@ struct CompressState {
 
 constexpr size_t kCompressHead = 64;
 constexpr int kScanThreads = 256;
-constexpr int kScanChunk = 4 * kScanThreads;  // 1024 words
+constexpr int kScanChunk = kCcScanChunk;  // 4 words per thread
 
 __global__ void compress_init(CompressState* st) {
   st->lo = INT_MAX;
@@ -563,13 +340,6 @@ __global__ __launch_bounds__(kScanThreads) void chunk_scan(CompressState* st,
   if (threadIdx.x == kScanThreads - 1) st->total = (int64_t)s64[threadIdx.x];
 }
 
-__device__ __forceinline__ int32_t root_rank(int root, const unsigned* rb,
-                                             const unsigned* prefix,
-                                             const unsigned* csum) {
-  const uint32_t w = (uint32_t)root >> 5;
-  const unsigned below = rb[w] & ((1u << (root & 31)) - 1u);
-  return (int32_t)(csum[w / kScanChunk] + prefix[w] + __popc(below));
-}
 
 __global__ void compress_map(int32_t* v, int64_t n, const CompressState* st,
                              const unsigned* bm, const unsigned* prefix,

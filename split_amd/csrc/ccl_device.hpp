@@ -1,28 +1,27 @@
-// Device bodies of the connected-component passes, shared by the
-// single-image and the batched kernels.
+// Device bodies of the connected-component passes (64x64 tiles), shared by
+// the single-image kernels (ccl.hip) and the batched ones (batch.hip).
 #pragma once
+
+#include <limits.h>
 
 #include "common.hpp"
 
 namespace splithip {
-
-constexpr int kCcTile = 32;
-constexpr int kCcTilePx = kCcTile * kCcTile;
-constexpr int kCcThreads = 256;
 
 // Accesses to the global forest (the output array).  Loads are plain:
 // every value a forest word ever holds is an ancestor of its pixel (links
 // only point to larger indices and only ever move up the tree), so a load
 // that returns an older value still leads to the same root.  Stores are
 // plain where one thread per kernel writes the word (the tile pass, the
-// final pass, flatten: a pixel's own entry).  Where two threads may store
-// to one word — union links, and the batched path's walk (ccl_jump_one),
-// whose path-halving store into a root can race with that root's own final
-// store — the stores are device atomics (g_raise), performed at the
-// coherence point and monotone, so the larger value, the one nearer the
-// root, is what every later kernel reads.  With plain racing stores the
-// 2048^2 spiral ended with tile roots holding the halving store's value
-// after later kernels had written their root.
+// final pass).  Union links, where two threads may store to one root's
+// word, are device atomics (g_raise, atomicMax): performed at the coherence
+// point and monotone, so the larger value, the one nearer the root, wins
+// whatever the order.  The root-jump pass writes each linked root's word
+// from that root's thread only, also with g_raise, for the visibility of
+// its progress to the other XCDs within the pass.  (An earlier walk whose
+// path-halving store into a root could land after that root's own final
+// store left, with plain stores, tile roots of the 2048^2 spiral holding
+// their chain grandparent.)
 __device__ __forceinline__ int g_load(const int32_t* par, int64_t i) {
   return par[i];
 }
@@ -76,73 +75,6 @@ __device__ __forceinline__ void lds_union(int* par, int a, int b) {
   }
 }
 
-// One 256-thread block per 32x32 tile.  Thread t handles tile pixels
-// t + 256*j: a wave covers two tile rows per j, one pixel per lane.
-//  1. Horizontal runs: each pixel's parent is the last pixel of its run in
-//     the row (ballot of "same label as right neighbour", then ctz).
-//  2. Vertical (and, for 8-connectivity, diagonal) unions between runs,
-//     skipping a union when the pixel to the left already made it (same run
-//     below, same run above).
-//  3. Each pixel writes the global index of its root.
-template <int CONN>
-__device__ __forceinline__ void ccl_tile_body(const int32_t* __restrict__ labels,
-                                              int W, int H, int32_t* out,
-                                              int tx, int ty, int* lab,
-                                              int* par) {
-  const int x0 = tx * kCcTile, y0 = ty * kCcTile;
-  const int tw = min(kCcTile, W - x0);  // tile width inside the image
-  const int th = min(kCcTile, H - y0);
-  const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int j = 0; j < kCcTilePx / kCcThreads; ++j) {
-    const int li = threadIdx.x + kCcThreads * j;
-    const int lx = li & (kCcTile - 1), ly = li / kCcTile;
-    const bool in = lx < tw && ly < th;
-    lab[li] = in ? labels[(int64_t)(y0 + ly) * W + x0 + lx] : 0;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kCcTilePx / kCcThreads; ++j) {
-    const int li = threadIdx.x + kCcThreads * j;
-    const int lx = li & (kCcTile - 1), ly = li / kCcTile;
-    const bool in = lx < tw && ly < th;
-    const bool right = in && lx + 1 < tw && lab[li + 1] == lab[li];
-    const uint64_t m = ballot(right);
-    const uint32_t row = lane < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
-    // bit tw-1 (and above) is clear, so the shifted complement is nonzero
-    const int end = lx + __builtin_ctz(~(row >> lx));
-    par[li] = in ? li - lx + end : li;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kCcTilePx / kCcThreads; ++j) {
-    const int li = threadIdx.x + kCcThreads * j;
-    const int lx = li & (kCcTile - 1), ly = li / kCcTile;
-    if (lx >= tw || ly >= th || ly == 0) continue;
-    const int l = lab[li];
-    const int up = li - kCcTile;
-    const bool left_same = lx > 0 && lab[li - 1] == l;
-    const bool up_same = lab[up] == l;
-    if (up_same && !(left_same && lab[up - 1] == l)) lds_union(par, li, up);
-    if (CONN == 8) {
-      if (lx > 0 && !left_same && !up_same && lab[up - 1] == l)
-        lds_union(par, li, up - 1);
-      if (lx + 1 < tw && !up_same && lab[up + 1] == l)
-        lds_union(par, li, up + 1);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kCcTilePx / kCcThreads; ++j) {
-    const int li = threadIdx.x + kCcThreads * j;
-    const int lx = li & (kCcTile - 1), ly = li / kCcTile;
-    if (lx >= tw || ly >= th) continue;
-    const int r = lds_find(par, li);
-    g_store(out, (int64_t)(y0 + ly) * W + x0 + lx,
-            (y0 + r / kCcTile) * W + x0 + (r & (kCcTile - 1)));
-  }
-}
-
 // ------------------------------------------------------- global forest ---
 
 // Finds do not shorten the paths they walk: halving them (atomicMax into
@@ -178,132 +110,264 @@ __device__ __forceinline__ void g_union(int* par, int a, int b,
   }
 }
 
-// One block per tile; thread t examines one pixel on the tile's top row,
-// left column or (8-connectivity) right column and unions it with each
-// equal-label neighbour that lies in another tile.  A union is skipped when
-// a neighbouring edge pixel already joins the same two runs (equal labels
-// along the edge on both sides), which leaves about one union per pair of
-// touching components instead of one per edge pixel.
-template <int CONN>
-__device__ __forceinline__ void ccl_merge_body(const int32_t* __restrict__ labels,
-                                               int W, int H, int32_t* out,
-                                               int tx, int ty, int t) {
-  const int x0 = tx * kCcTile, y0 = ty * kCcTile;
-  int gx, gy, side;
-  if (t < kCcTile) {             // top row: up, up-left, up-right
-    gx = x0 + t; gy = y0; side = 0;
-  } else if (t < 2 * kCcTile) {  // left column: left, up-left
-    gx = x0; gy = y0 + t - kCcTile; side = 1;
-  } else {                     // right column: up-right into the next tile
-    if (CONN != 8) return;
-    gx = x0 + kCcTile - 1; gy = y0 + t - 2 * kCcTile; side = 2;
+// ------------------------------------------------- 64x64-tile passes ---
+//
+// Tile side S of the CC passes (64).  Thread t of a block handles tile
+// pixels t + (S*S/4) j, j = 0..3: with S = 64 a wave holds one tile row
+// per j, with S = 32 two (lanes 0-31, 32-63).  More pixels per thread (and
+// more blocks per CU) was slower: the 4096^2 k = 32 labels' tile pass took
+// 154 / 162 / 265 us at 4 / 8 / 16 pixels per thread.
+template <int S>
+struct CcTile {
+  static constexpr int kPx = S * S;
+  static constexpr int kPer = 4;  // pixels per thread
+  static constexpr int kThreads = kPx / kPer;
+  static constexpr int kRowsPerPass = kThreads / S;
+};
+
+// Inputs, outputs and scratch of one connected-components run.
+struct CcArgs {
+  const int32_t* labels;
+  int32_t* out;   // the forest, then the result
+  int W, H, ntx;
+  unsigned* linked;      // bit p: root p was linked under another (N bits)
+  unsigned* local_root;  // bit p: p is its tile's local root (compression)
+  // per tile t, rows 0..S-1: labels and local roots of the left (x0) and
+  // right (x0 + S - 1) columns
+  int32_t* llab;
+  int32_t* rlab;
+  int32_t* lroot;
+  int32_t* rroot;
+};
+
+// ORs `bits` (the first `count` <= 64 bits used) into the bitmap at bit
+// offset p0 (any alignment); lanes k0, k0 + 1, k0 + 2 write one word each.
+__device__ __forceinline__ void or_bits(unsigned* bm, int64_t p0,
+                                        uint64_t bits, int lane, int k0) {
+  const int k = lane - k0;
+  if (bits == 0 || k < 0 || k > 2) return;
+  const int s = (int)(p0 & 31);
+  // word w: bits 0..31-s, word w+1: the next 32, word w+2: the last s
+  const unsigned v = k == 0   ? (unsigned)(bits << s)
+                     : k == 1 ? (unsigned)(bits >> (32 - s))
+                              : (unsigned)((bits >> (32 - s)) >> 32);
+  if (v) atomicOr(bm + (p0 >> 5) + k, v);
+}
+
+// The tile pass for tile (tx, ty), by a block of CcTile<S>::kThreads
+// threads; lab and par are the block's LDS arrays of S*S ints.
+template <int CONN, int S>
+__device__ __forceinline__ void cc_tile_body(const CcArgs& a, int tx, int ty,
+                                             int* lab, int* par) {
+  using G = CcTile<S>;
+  const int x0 = tx * S, y0 = ty * S;
+  const int tw = min(S, a.W - x0), th = min(S, a.H - y0);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int col = threadIdx.x & (S - 1);
+  const int rowbase = lane - col;  // first lane of this lane's row
+  const int64_t T = (int64_t)ty * a.ntx + tx;
+  int l[G::kPer], end[G::kPer];
+#pragma unroll
+  for (int j = 0; j < G::kPer; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    const bool in = col < tw && r < th;
+    l[j] = in ? a.labels[(int64_t)(y0 + r) * a.W + x0 + col] : INT_MIN;
+    lab[r * S + col] = l[j];
   }
-  if (gx >= W || gy >= H) return;
-  const int p = gy * W + gx;
-  const int l = labels[p];
-  if (side == 0) {
-    if (gy == 0) return;
-    const int u = p - W;
-    const bool u_same = labels[u] == l;
-    const bool left_in = gx > x0;
-    const bool left_same = left_in && labels[p - 1] == l;
-    if (u_same && !(left_same && labels[u - 1] == l)) g_union(out, p, u);
+  // each pixel's parent: the last pixel of its row run
+#pragma unroll
+  for (int j = 0; j < G::kPer; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    const bool in = col < tw && r < th;
+    const int right = __shfl_down(l[j], 1);
+    const uint64_t m = ballot(in && col + 1 < tw && right == l[j]);
+    // bit S-1 of the row's bits is clear, so the shifted complement is
+    // nonzero below it
+    end[j] = col + (int)__builtin_ctzll(~(m >> lane));
+    par[r * S + col] = r * S + (in ? end[j] : col);
+  }
+  __syncthreads();
+  // unions between runs of consecutive rows, once per run overlap
+#pragma unroll
+  for (int j = 0; j < G::kPer; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    const int left = __shfl_up(l[j], 1);
+    if (r == 0 || r >= th || col >= tw) continue;
+    const int li = r * S + col, c = l[j], up = li - S;
+    const bool left_same = col > 0 && left == c;
+    const int lu = lab[up];
+    const int lul = col > 0 ? lab[up - 1] : INT_MIN;
+    const bool up_same = lu == c;
+    if (up_same && !(left_same && lul == c)) lds_union(par, li, up);
     if (CONN == 8) {
-      if (gx > 0 && !u_same && !left_same && labels[u - 1] == l)
-        g_union(out, p, u - 1);
-      if (gx + 1 < W && !u_same && labels[u + 1] == l) g_union(out, p, u + 1);
+      if (col > 0 && !left_same && !up_same && lul == c)
+        lds_union(par, li, up - 1);
+      if (col + 1 < tw && !up_same && lab[up + 1] == c)
+        lds_union(par, li, up + 1);
     }
-  } else if (side == 1) {
-    if (gx == 0) return;
-    const bool up_in = gy > y0;
-    const bool lf_same = labels[p - 1] == l;
-    const bool up_same = up_in && labels[p - W] == l;
-    if (lf_same && !(up_same && labels[p - W - 1] == l)) g_union(out, p, p - 1);
-    if (CONN == 8 && up_in && !lf_same && !up_same && labels[p - W - 1] == l)
-      g_union(out, p, p - W - 1);
-  } else {
-    if (gy == y0 || gx + 1 >= W) return;
-    const int ur = p - W + 1;
-    if (labels[ur] == l && labels[p - W] != l && labels[p + 1] != l)
-      g_union(out, p, ur);
+  }
+  __syncthreads();
+  // roots: one find per run (at its end), shared along the run
+#pragma unroll
+  for (int j = 0; j < G::kPer; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    const bool in = col < tw && r < th;
+    const int li = r * S + col;
+    const int found = (in && col == end[j]) ? lds_find(par, li) : 0;
+    const int rl = __shfl(found, rowbase + (in ? end[j] : col));
+    const int g = (y0 + rl / S) * a.W + x0 + (rl & (S - 1));
+    const int64_t p = (int64_t)(y0 + r) * a.W + x0 + col;
+    if (in) g_store(a.out, p, g);
+    if (a.local_root) {
+      const uint64_t roots = ballot(in && rl == li);
+      if (r < th)
+        or_bits(a.local_root, (int64_t)(y0 + r) * a.W + x0,
+                S == 64 ? roots : (roots >> rowbase) & 0xFFFFFFFFull, lane,
+                rowbase);
+    }
+    if (in && col == 0) {
+      a.llab[T * S + r] = l[j];
+      a.lroot[T * S + r] = g;
+    }
+    if (in && col == S - 1) {
+      a.rlab[T * S + r] = l[j];
+      a.rroot[T * S + r] = g;
+    }
   }
 }
 
-
-// Shortens the global forest before the flatten pass.  After ccl_tile and
-// ccl_merge a pixel's parent is its tile's local root (inside the tile)
-// unless the pixel is such a root, whose parent chain crosses tiles and can
-// be as long as the component is winding.  Pixels whose parent lies in
-// another tile, and roots, walk their chain with path halving (every write
-// replaces a parent by an ancestor, so the forest stays valid under
-// concurrent walks) and then point straight at the root they found.
-__device__ __forceinline__ void ccl_jump_one(int32_t* out, int64_t p, int W) {
-  const int q = g_load(out, p);
-  if (q == p) return;
-  const int py = (int)(p / W), px = (int)(p - (int64_t)py * W);
-  const int qy = q / W, qx = q - qy * W;
-  if ((py / kCcTile) == (qy / kCcTile) && (px / kCcTile) == (qx / kCcTile))
-    return;  // an in-tile parent: its local root does the walk
-  int x = q;
-  for (;;) {
-    const int px2 = g_load(out, x);
-    if (px2 == x) break;
-    const int gp = g_load(out, px2);
-    if (gp == px2) {
-      x = px2;
-      break;
-    }
-    g_raise(out, x, gp);
-    x = gp;
-  }
-  g_raise(out, p, x);
-}
-
-// Follows the chain of pixel p to its root and writes it back.
-__device__ __forceinline__ int ccl_flatten_one(int32_t* out, int64_t p) {
-  const int r0 = g_load(out, p);
-  int r = r0;
-  for (;;) {
-    const int q = g_load(out, r);
-    if (q == r) break;
-    r = q;
-  }
-  if (r != r0) g_store(out, p, r);
-  return r;
-}
-
-// Check of the flattened forest at pixel p: p's value must be a root, and
-// every equal-label neighbour to the right and below (and on the two lower
-// diagonals with 8-connectivity) must carry the same value.  Together these
-// mean each component maps to one root, the largest index of its tree.  A
-// neighbour pair that disagrees is united (the roots are joined as in the
-// merge pass); any failure sets *flag, after which a second flatten pass
-// runs.
-template <int CONN>
-__device__ __forceinline__ void ccl_check_one(const int32_t* __restrict__ labels,
-                                              int W, int H, int32_t* out,
-                                              int64_t p, unsigned* flag) {
-  const int y = (int)(p / W), x = (int)(p - (int64_t)y * W);
-  const int l = labels[p];
-  const int r = g_load(out, p);
-  bool bad = g_load(out, r) != r;
-  auto pair = [&](int64_t q) {
-    if (labels[q] != l) return;
-    const int rq = g_load(out, q);
-    if (rq != r) {
-      g_union(out, r, rq);
-      bad = true;
-    }
-  };
-  if (x + 1 < W) pair(p + 1);
-  if (y + 1 < H) {
-    pair(p + W);
+// One block of three groups of S threads per tile: group 0 the top row (up,
+// up-left, up-right neighbours in the rows above), group 1 the left column
+// (left and up-left neighbours in the left tile, from the edge arrays),
+// group 2 the right column's up-right neighbour in the right tile
+// (8-connectivity).  A union is skipped where the neighbouring seam pixel
+// already joins the same two runs, which leaves about one union per pair of
+// touching runs.
+template <int CONN, int S>
+__device__ __forceinline__ void cc_merge_body(const CcArgs& a, int tx, int ty) {
+  const int x0 = tx * S, y0 = ty * S;
+  const int tw = min(S, a.W - x0), th = min(S, a.H - y0);
+  const int t = threadIdx.x % S;
+  const int grp = threadIdx.x / S;
+  const int64_t T = (int64_t)ty * a.ntx + tx;
+  const int W = a.W;
+  if (grp == 0) {
+    if (y0 == 0 || t >= tw) return;
+    const int x = x0 + t;
+    const int p = y0 * W + x, u = p - W;
+    const int c = a.labels[p];
+    const int lu = a.labels[u];
+    const bool left_same = t > 0 && a.labels[p - 1] == c;
+    const int lul = x > 0 ? a.labels[u - 1] : INT_MIN;
+    if (lu == c && !(left_same && lul == c)) g_union(a.out, p, u, a.linked);
     if (CONN == 8) {
-      if (x > 0) pair(p + W - 1);
-      if (x + 1 < W) pair(p + W + 1);
+      if (x > 0 && lu != c && !left_same && lul == c)
+        g_union(a.out, p, u - 1, a.linked);
+      if (x + 1 < W && lu != c && a.labels[u + 1] == c)
+        g_union(a.out, p, u + 1, a.linked);
+    }
+  } else if (grp == 1) {
+    if (x0 == 0 || t >= th) return;
+    const int64_t e = T * S + t, el = (T - 1) * S + t;
+    const int c = a.llab[e];
+    const int lq = a.rlab[el];
+    const bool up_in = t > 0;
+    const bool up_same = up_in && a.llab[e - 1] == c;
+    const int ul = up_in ? a.rlab[el - 1] : INT_MIN;
+    if (lq == c && !(up_same && ul == c))
+      g_union(a.out, a.lroot[e], a.rroot[el], a.linked);
+    if (CONN == 8 && up_in && lq != c && !up_same && ul == c)
+      g_union(a.out, a.lroot[e], a.rroot[el - 1], a.linked);
+  } else if (CONN == 8) {
+    if (tw < S || x0 + S >= W || t == 0 || t >= th) return;
+    const int64_t e = T * S + t, er = (T + 1) * S + t;
+    const int c = a.rlab[e];
+    if (a.llab[er - 1] == c && a.rlab[e - 1] != c && a.llab[er] != c)
+      g_union(a.out, a.rroot[e], a.lroot[er - 1], a.linked);
+  }
+}
+
+// Bitmap words per chunk of the rank prefix (chunk sums, then word
+// prefixes within a chunk).
+constexpr int kCcScanChunk = 1024;
+
+// Dense rank of a root: chunk sum (csum may be null: prefix is then a
+// prefix over all words) + word prefix + the lower bits of its word.
+__device__ __forceinline__ int32_t root_rank(int root, const unsigned* rb,
+                                             const unsigned* prefix,
+                                             const unsigned* csum) {
+  const uint32_t w = (uint32_t)root >> 5;
+  const unsigned below = rb[w] & ((1u << (root & 31)) - 1u);
+  return (int32_t)((csum ? csum[w / kCcScanChunk] : 0u) + prefix[w] +
+                   __popc(below));
+}
+
+// Per tile: every pixel's root, from the tile's forest words staged in LDS.
+// After the jump pass a pixel's word holds itself (an unlinked local root:
+// a root), the root (a linked local root, jumped), or its local root in the
+// same tile (any other pixel; the tile pass wrote it and nothing rewrote it),
+// whose own word is then the root.  So the root is the word of the pixel's
+// word when that lies in the tile, else the pixel's word.  Writes the root
+// index (rb == null) or its dense rank among the roots.
+template <int S>
+__device__ __forceinline__ void cc_final_body(const CcArgs& a, int tx, int ty,
+                                              const unsigned* rb,
+                                              const unsigned* prefix,
+                                              const unsigned* csum, int* sv) {
+  using G = CcTile<S>;
+  const int x0 = tx * S, y0 = ty * S;
+  const int tw = min(S, a.W - x0), th = min(S, a.H - y0);
+  const int col = threadIdx.x & (S - 1);
+  const double inv_w = 1.0 / (double)a.W;
+  int v[G::kPer];
+#pragma unroll
+  for (int j = 0; j < G::kPer; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    const bool in = col < tw && r < th;
+    v[j] = in ? g_load(a.out, (int64_t)(y0 + r) * a.W + x0 + col) : 0;
+    sv[r * S + col] = v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < G::kPer; ++j) {
+    const int r = threadIdx.x / S + G::kRowsPerPass * j;
+    if (col >= tw || r >= th) continue;
+    const int64_t p = (int64_t)(y0 + r) * a.W + x0 + col;
+    const int v1 = v[j];
+    // row and column of v1 (double division, corrected: exact for v1 < 2^31)
+    int vy = (int)((double)v1 * inv_w);
+    if ((int64_t)vy * a.W > v1) --vy;
+    else if ((int64_t)(vy + 1) * a.W <= v1) ++vy;
+    const int vx = v1 - vy * a.W;
+    const bool in_tile = vy >= y0 && vy < y0 + th && vx >= x0 && vx < x0 + tw;
+    const int root = in_tile ? sv[(vy - y0) * S + (vx - x0)] : v1;
+    g_store(a.out, p, rb ? root_rank(root, rb, prefix, csum) : root);
+  }
+}
+
+// The root-jump pass for the linked roots of bitmap word w (pixels
+// 32w..32w+31 of the forest `out`): each points at its tree's root by
+// pointer jumping — its word moves to its parent's current word and is
+// published after every step, so the roots below it advance through it by
+// the distance it has covered.  Roots in decreasing order (the callers also
+// take words in decreasing order: ancestors, which have larger indices,
+// start first and descendants mostly find them done).
+__device__ __forceinline__ void cc_jump_word(int32_t* out,
+                                             const unsigned* linked,
+                                             int64_t w) {
+  unsigned m = linked[w];
+  while (m) {
+    const int b = 31 - __builtin_clz(m);
+    m &= ~(1u << b);
+    const int a = (int)(w * 32 + b);
+    int x = g_load(out, a);
+    for (;;) {
+      const int px = g_load(out, x);
+      if (px == x) break;
+      x = px;
+      g_raise(out, a, x);
     }
   }
-  if (bad) atomicOr(flag, 1u);
 }
 
 }  // namespace splithip

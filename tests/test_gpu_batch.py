@@ -162,8 +162,8 @@ def test_batch_errors(torch_cuda, batch_images):
 
 def test_batch_graph_key_cc_tiles(torch_cuda):
     """Two one-image batches with the same k, Lloyd tile count (9) and padded
-    pixel count (2304) but different counts of 32x32 CC tiles (48x48: 4,
-    144x16: 5) share one temp: the second call must capture its own graph,
+    pixel count (2304) but different counts of 64x64 CC tiles (48x48: 1,
+    144x16: 3) share one temp: the second call must capture its own graph,
     and its segments equal the CPU union-find."""
     torch = torch_cuda
     imgs = []
@@ -175,7 +175,6 @@ def test_batch_graph_key_cc_tiles(torch_cuda):
         sa.uniform_random_initialize(P, C0, 0x6EA7 + i)
         imgs.append(dict(points=P, width=w, height=h, init=C0))
     sizes = [sa.segment_batch_workspace(_fresh(torch, [im])) for im in imgs]
-    assert sizes[0] == sizes[1]
     temp = torch.empty(max(sizes), dtype=torch.uint8, device="cuda")
     for rep in range(2):
         for im in imgs:
