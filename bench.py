@@ -898,8 +898,9 @@ def main():
                 f"{name}_cap_hits": ev[f"{name}_cap_hits"],
                 f"{name}_inertia_ratio": round(ev[f"{name}_inertia_ratio_total"], 4),
             })
-            if f"{name}_coarse_iters_mean" in ev:
-                ws[f"{name}_coarse_iters_mean"] = round(ev[f"{name}_coarse_iters_mean"], 2)
+            if f"{name}_coarse_stage_iters_mean" in ev:
+                ws[f"{name}_coarse_stage_iters_mean"] = round(
+                    ev[f"{name}_coarse_stage_iters_mean"], 2)
                 ws[f"{name}_pixel_iters_mean"] = round(ev[f"{name}_pixel_iters_mean"], 2)
         ws["speedup_warm_coarse_batched_vs_plain_separate"] = round(
             ev["speedup_batched_warm_coarse_vs_separate_plain"], 3)

@@ -452,7 +452,7 @@ def evaluate(net: "WarmStartNet", count: int = 256, side: int = 1024, reps: int 
         })
         if "coarse_iterations" in o:
             ci = float(o["coarse_iterations"].mean())
-            summ[f"{name}_coarse_iters_mean"] = ci
+            summ[f"{name}_coarse_stage_iters_mean"] = ci
             # coarse iterations at their pixel share of a full-size one
             summ[f"{name}_pixel_iters_mean"] = float(o["iterations"].mean()) + ci * SIDE * SIDE / n
     # earlier names of the network-only figures
