@@ -31,9 +31,9 @@
 // The whole sequence is captured once into a hipGraph per (temp, batch
 // shape, settings) and replayed; per-image buffer pointers live in a
 // descriptor table in temp that is uploaded before each replay.
-// A small autotuner picks the step kernel's block size and blocks per CU
-// by timing one iteration of each candidate on the batch itself, cached per
-// batch signature.
+// An autotuner picks the Lloyd tile shape (16x16, 32x8 or 64x4) and the
+// step kernels' block size and blocks per CU by timing iterations of each
+// candidate on the batch itself, cached per batch signature.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -76,6 +76,7 @@ struct BatchImg {
   unsigned maxabs[3];  // largest |v| of each channel (float bits)
   int tiles_done;
   int bounded;  // iterations after the first are bounded (keep the totals)
+  int tw_log;   // Lloyd tile shape (Geometry::tw_log), the same in a batch
 };
 
 struct BatchArgs {
@@ -176,8 +177,9 @@ __device__ __forceinline__ Geometry image_geometry(const BatchImg& m) {
   g.width = m.W;
   g.height = m.H;
   g.full_rows = m.H;
+  g.tw_log = m.tw_log;
   g.tiles_x = m.tiles_x;
-  g.nseg = (int64_t)m.tiles_x * ((m.H + kTileSide - 1) / kTileSide);
+  g.nseg = (int64_t)m.tiles_x * tiles_down(m.H, g.tw_log);
   return g;
 }
 
@@ -717,7 +719,7 @@ struct BatchPlan {
 };
 
 static int make_plan(const split_hip_image_desc* d, int count, void* temp,
-                     bool need_segments, BatchPlan* P) {
+                     bool need_segments, int tw_log, BatchPlan* P) {
   for (int i = 0; i < count; ++i) {
     const split_hip_image_desc& e = d[i];
     if (e.width < 1 || e.height < 1 || (int64_t)e.width * e.height > INT32_MAX)
@@ -756,12 +758,13 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
     m.H = e.height;
     m.k = e.k;
     m.bounded = j >= P->nplain;
-    m.tiles_x = (e.width + kTileSide - 1) / kTileSide;
+    m.tw_log = tw_log;
+    m.tiles_x = (int)tiles_across(e.width, tw_log);
     m.ctiles_x = (e.width + kCcSide - 1) / kCcSide;
     m.tile0 = tiles;
     m.ctile0 = ctiles;
     m.px0 = px;
-    tiles += (int64_t)m.tiles_x * ((e.height + kTileSide - 1) / kTileSide);
+    tiles += (int64_t)m.tiles_x * tiles_down(e.height, tw_log);
     const int64_t img_ctiles =
         (int64_t)m.ctiles_x * ((e.height + kCcSide - 1) / kCcSide);
     ctiles += img_ctiles;
@@ -822,34 +825,55 @@ static void launch_step(const BatchArgs& a, const Group& g, bool bounded,
                        lds, stream, a, g);
 }
 
-// Autotune cache: (device, group kind, images, tiles, largest k) -> best
-// step configuration.
-using TuneKey = std::tuple<int, int, int, int64_t, int>;
+// Autotuning of the Lloyd tile shape and the step kernels' block shapes,
+// cached per batch signature (device, images, padded pixels, plain images,
+// largest k of each group).  Every tile shape gives the same results (exact
+// argmin, integer sums); for each shape the tables are re-planned and each
+// group's block size x blocks-per-CU candidates are timed over one full
+// and three bounded iterations of the batch itself (after one untimed full
+// iteration); the shape with the smallest sum of its groups' best times
+// wins.  batch_256: 16x16 / 32x8 / 64x4 tiles took 178.8 / 171.8 / 176.9 ms
+// per batch (bounded iteration 2.40 / 2.20 / 2.23 ms).
+struct BatchTune {
+  int tw_log = 4;
+  StepTune tp, tb;
+};
+using BatchTuneKey = std::tuple<int, int, int64_t, int, int, int>;
 static std::mutex g_tune_mu;
-static std::map<TuneKey, StepTune> g_tune;
+static std::map<BatchTuneKey, BatchTune> g_tune;
 
-// Times 3 full-assignment iterations of the group for each block size and
-// blocks-per-CU candidate (up to the occupancy limit).
-static int autotune_group(const BatchArgs& a, const Group& g, bool bounded,
-                          int kmax, int device, hipStream_t stream,
-                          StepTune* out) {
-  const TuneKey key{device, bounded ? 1 : 0, g.count, g.t1 - g.t0, kmax};
-  {
-    std::lock_guard<std::mutex> lock(g_tune_mu);
-    auto it = g_tune.find(key);
-    if (it != g_tune.end()) {
-      *out = it->second;
-      return SPLIT_HIP_OK;
-    }
-  }
-  hipEvent_t e0, e1;
-  SPLIT_HIP_CHECK(hipEventCreate(&e0));
-  SPLIT_HIP_CHECK(hipEventCreate(&e1));
-  StepTune best;
-  float best_ms = 1e30f;
+constexpr int kTileShapes[] = {4, 5, 6};  // tw_log: 16x16, 32x8, 64x4
+
+static int time_group(const BatchArgs& a, const Group& g, bool bounded,
+                      const StepTune& cand, hipEvent_t e0, hipEvent_t e1,
+                      hipStream_t stream, float* ms) {
   BatchArgs ta = a;
   ta.max_iter = 1 << 30;  // never stop while timing
   ta.threshold = -1.f;
+  hipLaunchKernelGGL(batch_init, dim3(a.count), dim3(256), 0, stream, ta);
+  hipLaunchKernelGGL(batch_absmax, dim3(8 * device_cu_count()), dim3(256), 0,
+                     stream, ta);
+  launch_step(ta, g, bounded, true, cand, stream);  // untimed
+  SPLIT_HIP_CHECK(hipEventRecord(e0, stream));
+  launch_step(ta, g, bounded, true, cand, stream);
+  for (int r = 0; r < 3; ++r) {
+    // a full step re-assigns (and records) every pixel: the bounded steps
+    // after it run from a consistent state
+    launch_step(ta, g, bounded, false, cand, stream);
+  }
+  SPLIT_HIP_CHECK(hipEventRecord(e1, stream));
+  SPLIT_HIP_CHECK(hipEventSynchronize(e1));
+  SPLIT_HIP_CHECK(hipEventElapsedTime(ms, e0, e1));
+  SPLIT_HIP_CHECK_LAUNCH("batch autotune");
+  return SPLIT_HIP_OK;
+}
+
+// Best block shape of one group on the uploaded tables, and its time.
+static int tune_group(const BatchArgs& a, const Group& g, bool bounded,
+                      hipEvent_t e0, hipEvent_t e1, hipStream_t stream,
+                      StepTune* out, float* out_ms) {
+  StepTune best;
+  float best_ms = 1e30f;
   const int cands_plain[] = {256, 512, 1024};
   const int cands_bounded[] = {256, 512};
   const int* cands = bounded ? cands_bounded : cands_plain;
@@ -864,32 +888,19 @@ static int autotune_group(const BatchArgs& a, const Group& g, bool bounded,
       occ = 1;
     for (int per_cu = 1; per_cu <= occ; per_cu *= 2) {
       const StepTune cand{threads, per_cu};
-      hipLaunchKernelGGL(batch_init, dim3(a.count), dim3(256), 0, stream, ta);
-      hipLaunchKernelGGL(batch_absmax, dim3(8 * device_cu_count()), dim3(256),
-                         0, stream, ta);
-      launch_step(ta, g, bounded, true, cand, stream);  // warm-up
-      SPLIT_HIP_CHECK(hipEventRecord(e0, stream));
-      for (int r = 0; r < 3; ++r) launch_step(ta, g, bounded, true, cand, stream);
-      SPLIT_HIP_CHECK(hipEventRecord(e1, stream));
-      SPLIT_HIP_CHECK(hipEventSynchronize(e1));
       float ms = 0.f;
-      SPLIT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      SPLIT_HIP_TRY(time_group(a, g, bounded, cand, e0, e1, stream, &ms));
       if (ms < best_ms) {
         best_ms = ms;
         best = cand;
       }
     }
   }
-  SPLIT_HIP_CHECK_LAUNCH("batch autotune");
-  SPLIT_HIP_CHECK(hipEventDestroy(e0));
-  SPLIT_HIP_CHECK(hipEventDestroy(e1));
-  {
-    std::lock_guard<std::mutex> lock(g_tune_mu);
-    g_tune[key] = best;
-  }
   *out = best;
+  *out_ms = best_ms;
   return SPLIT_HIP_OK;
 }
+
 
 struct Groups {
   Group plain, bounded;
@@ -941,12 +952,13 @@ struct GraphEntry {
 };
 // Everything a captured graph bakes in: (device, temp, images, plain
 // images, plain tiles, Lloyd tiles, CC tiles, pixels, max_iter, threshold,
-// connectivity, compress, tuned plain / bounded shapes).  Per-image sizes,
+// connectivity, compress, tuned plain / bounded block shapes, tile shape).
+// Per-image sizes,
 // k and buffers are read from the descriptor table at run time; the totals
 // are launch bounds and grid sizes.
 using GraphKey = std::tuple<int, void*, int, int, int64_t, int64_t, int64_t,
                             int64_t, int, float, int, int, int, int, int,
-                            int>;
+                            int, int>;
 static std::mutex g_graph_mu;
 static std::map<GraphKey, GraphEntry> g_graphs;
 
@@ -960,7 +972,7 @@ size_t split_hip_segment_batch_workspace(const split_hip_image_desc* images,
                                          int32_t count) {
   if (!images || count < 1) return 0;
   BatchPlan P;
-  if (make_plan(images, count, nullptr, false, &P) != SPLIT_HIP_OK) return 0;
+  if (make_plan(images, count, nullptr, false, 4, &P) != SPLIT_HIP_OK) return 0;
   return P.total_bytes;
 }
 
@@ -977,39 +989,84 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
   if (connectivity != 0 && connectivity != 4 && connectivity != 8)
     return fail(SPLIT_HIP_ERR_ARGUMENT, "connectivity %d", connectivity);
   BatchPlan P;
-  SPLIT_HIP_TRY(make_plan(images, count, temp, connectivity != 0, &P));
+  SPLIT_HIP_TRY(make_plan(images, count, temp, connectivity != 0, 4, &P));
   if (temp_bytes < P.total_bytes)
     return fail(SPLIT_HIP_ERR_TEMP, "temp holds %zu bytes, batch needs %zu",
                 temp_bytes, P.total_bytes);
   BatchArgs a;
   a.img = static_cast<BatchImg*>(temp);
   a.count = count;
-  a.tiles = P.tiles;
-  a.ctiles = P.ctiles;
-  a.px = P.px;
   a.max_iter = std::max(0, max_iter);
   a.threshold = threshold;
-  SPLIT_HIP_CHECK(hipMemcpyAsync(temp, P.img.data(),
-                                 (size_t)count * sizeof(BatchImg),
-                                 hipMemcpyHostToDevice, stream));
   int device = 0;
   SPLIT_HIP_CHECK(hipGetDevice(&device));
-  Groups G;
-  G.plain = Group{a.img, P.nplain, 0, P.tiles_plain};
-  G.bounded = Group{a.img + P.nplain, count - P.nplain, P.tiles_plain, P.tiles};
   int kp = 0, kb = 0;
   for (int j = 0; j < count; ++j)
     (j < P.nplain ? kp : kb) = std::max(j < P.nplain ? kp : kb, P.img[j].k);
-  if (G.plain.count)
-    SPLIT_HIP_TRY(autotune_group(a, G.plain, false, kp, device, stream, &G.tp));
-  if (G.bounded.count)
-    SPLIT_HIP_TRY(autotune_group(a, G.bounded, true, kb, device, stream, &G.tb));
+  Groups G;
+  // tables for tile shape tw (re-planned unless they have it), uploaded
+  auto upload = [&](int tw) -> int {
+    if (P.img[0].tw_log != tw)
+      SPLIT_HIP_TRY(make_plan(images, count, temp, connectivity != 0, tw, &P));
+    a.tiles = P.tiles;
+    a.ctiles = P.ctiles;
+    a.px = P.px;
+    SPLIT_HIP_CHECK(hipMemcpyAsync(temp, P.img.data(),
+                                   (size_t)count * sizeof(BatchImg),
+                                   hipMemcpyHostToDevice, stream));
+    G.plain = Group{a.img, P.nplain, 0, P.tiles_plain};
+    G.bounded = Group{a.img + P.nplain, count - P.nplain, P.tiles_plain, P.tiles};
+    return SPLIT_HIP_OK;
+  };
+  const BatchTuneKey tkey{device, count, P.px, P.nplain, kp, kb};
+  BatchTune tune;
+  bool tuned = false;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find(tkey);
+    if (it != g_tune.end()) {
+      tune = it->second;
+      tuned = true;
+    }
+  }
+  if (!tuned) {
+    hipEvent_t e0, e1;
+    SPLIT_HIP_CHECK(hipEventCreate(&e0));
+    SPLIT_HIP_CHECK(hipEventCreate(&e1));
+    float best_total = 1e30f;
+    for (const int tw : kTileShapes) {
+      SPLIT_HIP_TRY(upload(tw));
+      BatchTune cand;
+      cand.tw_log = tw;
+      float total = 0.f, ms = 0.f;
+      if (G.plain.count) {
+        SPLIT_HIP_TRY(tune_group(a, G.plain, false, e0, e1, stream, &cand.tp, &ms));
+        total += ms;
+      }
+      if (G.bounded.count) {
+        SPLIT_HIP_TRY(tune_group(a, G.bounded, true, e0, e1, stream, &cand.tb, &ms));
+        total += ms;
+      }
+      if (total < best_total) {
+        best_total = total;
+        tune = cand;
+      }
+    }
+    SPLIT_HIP_CHECK(hipEventDestroy(e0));
+    SPLIT_HIP_CHECK(hipEventDestroy(e1));
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[tkey] = tune;
+  }
+  SPLIT_HIP_TRY(upload(tune.tw_log));
+  G.tp = tune.tp;
+  G.tb = tune.tb;
 
   if (use_graph) {
     const GraphKey key{device,      temp,          count,      P.nplain,
                        P.tiles_plain, P.tiles,     P.ctiles,   P.px,
                        a.max_iter,  threshold,     connectivity, compress != 0,
-                       G.tp.threads, G.tp.per_cu,  G.tb.threads, G.tb.per_cu};
+                       G.tp.threads, G.tp.per_cu,  G.tb.threads, G.tb.per_cu,
+                       tune.tw_log};
     hipGraphExec_t exec = nullptr;
     {
       std::lock_guard<std::mutex> lock(g_graph_mu);

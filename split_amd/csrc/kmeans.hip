@@ -499,8 +499,9 @@ static void set_geometry(StepArgs& a, int64_t n, int width, int height) {
     a.width = width;
     a.height = height;
     a.full_rows = height;
-    a.tiles_x = (width + kTileSide - 1) / kTileSide;
-    a.nseg = (int64_t)a.tiles_x * ((height + kTileSide - 1) / kTileSide);
+    a.tw_log = 4;
+    a.tiles_x = (int)tiles_across(width, a.tw_log);
+    a.nseg = (int64_t)a.tiles_x * tiles_down(height, a.tw_log);
   } else {
     a.width = a.height = a.tiles_x = a.full_rows = 0;
     a.nseg = (n + kSegPx - 1) / kSegPx;

@@ -193,7 +193,7 @@ struct TileRange {
 constexpr int kQueueCap = 512;      // per-wave queue of failing pixels
 // Entries after an image's last pixel state that the walkers' 16-byte lane
 // loads may read (and ignore); state buffers are allocated with them.
-constexpr int kStatePad = 16;
+constexpr int kStatePad = 64;
 // More failing pixels: re-assign the tile whole (its compact colour box keeps
 // ~7 candidates) rather than queue them (a queued batch evaluates nearly all
 // k).  On the 4096² k = 32 image and the 256-image batch: 64 / 96 / 128 /
@@ -282,10 +282,10 @@ __device__ __forceinline__ void walk_tiles(
     // right edge read on into the next row or the kStatePad entries after
     // the last pixel; visit() masks such pixels.
     auto load_state = [&](const Pos& p, bool have, uint32_t (&ps)[4]) {
-      const int row = p.ty * kTileSide + (lane >> 2);
+      const int row = p.ty * tile_h(a) + lane_row(a, lane);
       const int64_t o =
-          have ? (int64_t)(row < a.height ? row : p.ty * kTileSide) * a.width +
-                     p.tx * kTileSide + 4 * (lane & 3)
+          have ? (int64_t)(row < a.height ? row : p.ty * tile_h(a)) * a.width +
+                     p.tx * tile_w(a) + lane_col(a, lane)
                : 0;
       const U4 q = *gmem(reinterpret_cast<const U4*>(a.pstate + o));
       ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
@@ -297,11 +297,11 @@ __device__ __forceinline__ void walk_tiles(
       bool fail[4];
       uint64_t fb[4];
       int nfail = 0;
-      const int row = p.ty * kTileSide + (lane >> 2);
-      const int col = p.tx * kTileSide + 4 * (lane & 3);
+      const int row = p.ty * tile_h(a) + lane_row(a, lane);
+      const int col = p.tx * tile_w(a) + lane_col(a, lane);
       const int64_t o = (int64_t)row * a.width + col;
-      if ((p.ty + 1) * kTileSide <= a.full_rows &&
-          (p.tx + 1) * kTileSide <= a.width) {
+      if ((p.ty + 1) * tile_h(a) <= a.full_rows &&
+          (p.tx + 1) * tile_w(a) <= a.width) {
         // every pixel exists: no per-lane guards
         lvec = MODE == kTileVec;
 #pragma unroll

@@ -21,6 +21,9 @@ struct Geometry {
   // rows that hold `width` pixels: height for an image; n / width for a
   // point set walked as a width-16 image, whose last row may be partial
   int full_rows;
+  // tiles are (1 << tw_log) x (256 >> tw_log) pixels, tw_log in 4..6: 16x16,
+  // 32x8 or 64x4 (a lane holds 4 consecutive pixels of one tile row)
+  int tw_log = 4;
   // optional interleaved copy of the pixels ({x, y, z, 0} per pixel) for
   // scattered reads: one 16-byte load touches one cache line instead of
   // three (one per plane)
@@ -40,7 +43,31 @@ __device__ __forceinline__ int row_pixels(const Geometry& a, int row) {
 // Traversal modes: 1-D runs of 256 pixels (dword or float4 loads) or 16x16
 // tiles (dword or float4 loads per lane row).
 enum Mode { kRun = 0, kRunVec = 1, kTile = 2, kTileVec = 3 };
-constexpr int kTileSide = 16;
+constexpr int kTilePx = 256;
+
+// Tile shape of a Geometry and the lane layout within a tile: lane l holds
+// pixels 4 (l mod (w/4)) .. +3 of tile row l / (w/4).
+__host__ __device__ __forceinline__ int tile_w(const Geometry& a) {
+  return 1 << a.tw_log;
+}
+__host__ __device__ __forceinline__ int tile_h(const Geometry& a) {
+  return kTilePx >> a.tw_log;
+}
+__device__ __forceinline__ int lane_row(const Geometry& a, int lane) {
+  return lane >> (a.tw_log - 2);
+}
+__device__ __forceinline__ int lane_col(const Geometry& a, int lane) {
+  return 4 * (lane & ((1 << (a.tw_log - 2)) - 1));
+}
+// Tiles across and down a width x rows area.
+__host__ __device__ __forceinline__ int64_t tiles_across(int width, int tw_log) {
+  return (width + (1 << tw_log) - 1) >> tw_log;
+}
+__host__ __device__ __forceinline__ int64_t tiles_down(int64_t rows,
+                                                       int tw_log) {
+  const int th = kTilePx >> tw_log;
+  return (rows + th - 1) / th;
+}
 
 // Pixel indices of segment s for this lane: idx[i] of pixel i, v[i] whether
 // it exists, lvec whether the lane's four pixels are consecutive and exist
@@ -53,8 +80,8 @@ __device__ __forceinline__ void segment_index(const Geometry& a, int64_t s,
   if constexpr (MODE == kTile || MODE == kTileVec) {
     const int ty = (int)(s / a.tiles_x);
     const int tx = (int)(s - (int64_t)ty * a.tiles_x);
-    const int row = ty * kTileSide + (lane >> 2);
-    const int col = tx * kTileSide + (lane & 3) * 4;
+    const int row = ty * tile_h(a) + lane_row(a, lane);
+    const int col = tx * tile_w(a) + lane_col(a, lane);
     const int64_t o = (int64_t)row * a.width + col;
     const int rp = row_pixels(a, row);
 #pragma unroll

@@ -209,8 +209,8 @@ __device__ __forceinline__ void list_units(const TileArgs& a, int64_t q0,
     const int64_t u = (q0 + i) * a.unit_mul % a.nunits;
     const int ty = (int)(u / upr);
     const int c = (int)(u - (int64_t)ty * upr);
-    const int row0 = ty * kTileSide;
-    list[i] = make_int2(row0 * a.width + 2 * c * kTileSide, row0);
+    const int row0 = ty * tile_h(a);
+    list[i] = make_int2(row0 * a.width + 2 * c * tile_w(a), row0);
   }
 }
 
@@ -227,15 +227,15 @@ struct BoundedWalker {
   long long* slice;
   int* qidx;
   int* qold;
-  int lane, lane_off;  // lane l: row l/4, columns 4(l%4)..+3 of a tile
+  int lane, lane_off;  // lane l: row lane_row, columns lane_col..+3 of a tile
   IRun run;
   int qlen;            // wave-uniform
 
   __device__ __forceinline__ void tile_pixels(const TileGeo& g,
                                               int64_t (&idx)[4], bool (&v)[4],
                                               bool& lvec) const {
-    const int col = g.col0 + 4 * (lane & 3);
-    const int rp = row_pixels(a, g.row0 + (lane >> 2));
+    const int col = g.col0 + lane_col(a, lane);
+    const int rp = row_pixels(a, g.row0 + lane_row(a, lane));
     const int o = g.off + lane_off;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -247,18 +247,19 @@ struct BoundedWalker {
 
   // Tiles wholly inside the image (wave-uniform: the geometry is scalar).
   __device__ __forceinline__ bool inner(const TileGeo& g) const {
-    return g.row0 + kTileSide <= a.full_rows && g.col0 + kTileSide <= a.width;
+    return g.row0 + tile_h(a) <= a.full_rows && g.col0 + tile_w(a) <= a.width;
   }
 
   // One 16-byte load per lane on every path, so that the wait for this
   // tile's state (issued a tile earlier) never has to cover the next tile's
   // load as well.  Lanes below the image read the tile's first row, lanes
   // past its right edge read on into the next row or the padding after the
-  // last pixel (at most 15 entries, kStatePad); visit() masks such pixels.
+  // last pixel (fewer than a tile width, kStatePad); visit() masks such
+  // pixels.
   __device__ __forceinline__ void load_state(const TileGeo& g, bool have,
                                              uint32_t (&ps)[4]) const {
-    const bool rowok = g.row0 + (lane >> 2) < a.height;
-    const int o = have ? g.off + (rowok ? lane_off : 4 * (lane & 3)) : 0;
+    const bool rowok = g.row0 + lane_row(a, lane) < a.height;
+    const int o = have ? g.off + (rowok ? lane_off : lane_col(a, lane)) : 0;
     const U4 q = *reinterpret_cast<const U4*>(a.pstate + o);
     ps[0] = q.x; ps[1] = q.y; ps[2] = q.z; ps[3] = q.w;
   }
@@ -293,7 +294,7 @@ struct BoundedWalker {
   // Skip test of one tile; re-assigns it whole or queues its failing pixels.
   __device__ __forceinline__ void visit(const TileGeo& g,
                                         const uint32_t (&ps)[4]) {
-    const int col = g.col0 + 4 * (lane & 3);
+    const int col = g.col0 + lane_col(a, lane);
     int old[4];
     bool fail[4];
     uint64_t fb[4];
@@ -307,7 +308,7 @@ struct BoundedWalker {
         nfail += __builtin_popcountll(fb[i]);
       }
     } else {
-      const int rp = row_pixels(a, g.row0 + (lane >> 2));
+      const int rp = row_pixels(a, g.row0 + lane_row(a, lane));
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const bool v = col + i < rp;
@@ -365,11 +366,11 @@ struct BoundedWalker {
       const int off = readfirstlane_i(e.x), row0 = readfirstlane_i(e.y);
       const int col0 = off - row0 * a.width;
       g = TileGeo{off, row0, col0};
-      if (col0 + kTileSide < a.width) {
+      if (col0 + tile_w(a) < a.width) {
         pending = true;
-        pend_off = off + kTileSide;
+        pend_off = off + tile_w(a);
         pend_row0 = row0;
-        pend_col0 = col0 + kTileSide;
+        pend_col0 = col0 + tile_w(a);
       }
       return true;
     };
@@ -510,7 +511,7 @@ __global__ __launch_bounds__(CPL >= 2 ? 256 : 1024, CPL >= 2 ? 2 : 1) void tiles
     __shared__ unsigned unit_next;
     BoundedWalker<CPL, MODE> w{a,     cx,    cy,   cz,   cn,
                                fl,    qs,    margin, slice, qidx,
-                               qold,  lane,  (lane >> 2) * a.width + 4 * (lane & 3),
+                               qold,  lane,  lane_row(a, lane) * a.width + lane_col(a, lane),
                                IRun{0, 0, 0, 0, -1, 0}, 0};
     const int64_t G = gridDim.x;
     const int64_t r0 = a.nunits * blockIdx.x / G;
@@ -686,12 +687,11 @@ static int tiles_threads(int k) { return k <= 64 ? 1024 : 256; }
 // Step launches use at most this many blocks.
 static int tiles_max_blocks(int k) { return k <= 64 ? 512 : 1024; }
 // Units (pairs of adjacent tiles) of n pixels in rows of width.
-static int64_t tile_units(int64_t n, int width) {
+static int64_t tile_units(int64_t n, int width, int tw_log) {
   if (width <= 0) return 0;
-  const int64_t tiles_x = (width + kTileSide - 1) / kTileSide;
+  const int64_t tiles_x = tiles_across(width, tw_log);
   const int64_t rows = (n + width - 1) / width;
-  const int64_t tiles_y = (rows + kTileSide - 1) / kTileSide;
-  return (tiles_x + 1) / 2 * tiles_y;
+  return (tiles_x + 1) / 2 * tiles_down(rows, tw_log);
 }
 
 struct TileLayout {
@@ -817,8 +817,13 @@ int cluster_tiles_async(const float* pts, int64_t n, int width,
   a.width = width;
   a.height = height;
   a.full_rows = (int)(n / width);
-  a.tiles_x = (width + kTileSide - 1) / kTileSide;
-  a.nseg = (int64_t)a.tiles_x * ((height + kTileSide - 1) / kTileSide);
+  // 32x8 tiles for k <= 64 on images at least 32 wide, else 16x16: on the
+  // 4096^2 k = 32 image 69.3 / 68.4 / 70.2 us per iteration with 16x16 /
+  // 32x8 / 64x4 tiles, on the 8192^2 k = 256 one 1.72 / 1.80 / 1.93 ms
+  // (the wider tiles' colour boxes keep more of 256 candidates)
+  a.tw_log = k <= 64 && width >= 32 ? 5 : 4;
+  a.tiles_x = (int)tiles_across(width, a.tw_log);
+  a.nseg = (int64_t)a.tiles_x * tiles_down(height, a.tw_log);
   a.k = k;
   a.cbuf = L.cbuf;
   a.totals = L.totals;
@@ -829,7 +834,7 @@ int cluster_tiles_async(const float* pts, int64_t n, int width,
   a.maxabs = L.maxabs;
   a.state = L.st;
   a.mbox = res->dev;
-  a.nunits = tile_units(n, width);
+  a.nunits = tile_units(n, width, a.tw_log);
   // measured on the 8192^2 k=256 image (every failing tile whole: 2.49 ms
   // per iteration; at 16 / 64 / 160 failing pixels: 2.34 / 1.83 / 1.76 ms)
   // and the 4096^2 k=32 one (77.7 / 73.4 / 71.0 / 70.1 us)

@@ -161,10 +161,11 @@ def test_batch_errors(torch_cuda, batch_images):
 
 
 def test_batch_graph_key_cc_tiles(torch_cuda):
-    """Two one-image batches with the same k, Lloyd tile count (9) and padded
-    pixel count (2304) but different counts of 64x64 CC tiles (48x48: 1,
-    144x16: 3) share one temp: the second call must capture its own graph,
-    and its segments equal the CPU union-find."""
+    """Two one-image batches with the same k and padded pixel count (2304)
+    (and, with 16x16 Lloyd tiles, the same tile count, 9) but different
+    counts of 64x64 CC tiles (48x48: 1, 144x16: 3) share one temp: the
+    second call must capture its own graph, and its segments equal the CPU
+    union-find."""
     torch = torch_cuda
     imgs = []
     for i, (w, h) in enumerate([(48, 48), (144, 16)]):
