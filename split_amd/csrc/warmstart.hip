@@ -9,7 +9,7 @@
 //   between the 16 predicted and 16 target colours
 //
 // Every layer — forward, weight gradient and input gradient — is one
-// implicit GEMM on the matrix cores (v_mfma_f32_16x16x16_bf16: bf16
+// implicit GEMM on the matrix cores (v_mfma_f32_16x16x32_bf16: bf16
 // operands, fp32 accumulation).  The GEMM kernel is generic over three
 // functors: the A and B loaders gather their operand tiles straight from the
 // activation / weight / gradient tensors (im2col and its transposes are
@@ -30,13 +30,15 @@
 namespace splithip {
 namespace ws {
 
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kSide = 256;
 constexpr int kOut = 3 * SPLIT_HIP_WS_K;  // 48 outputs
 constexpr int kBM = 64, kBK = 32, kThreads = 256;
-constexpr int kLdsPad = 4;  // shorts of padding per LDS row
+// shorts of padding per LDS row: rows stay 16-byte aligned for the MFMA
+// operands' 16-byte LDS reads (a row is 80 bytes)
+constexpr int kLdsPad = 8;
 constexpr int kMaxSplits = 256;
 
 struct LayerDim {
@@ -338,10 +340,12 @@ struct EpPartial {
 // of [0, R).  Block: 256 threads = 4 waves; tile 64 x BN; wave w owns rows
 // 16w..16w+15 and every 16-column subtile.  Per 32-deep step each thread
 // loads 8 A values (row t/4, columns 8*(t%4)..) and up to 8 B values, rounds
-// them to bf16 into LDS, then each wave issues 2 MFMAs per subtile.
+// them to bf16 into LDS, then each wave issues one gfx950
+// v_mfma_f32_16x16x32_bf16 per subtile.
 //
-// v_mfma_f32_16x16x16_bf16 lane layout: A[i][k] in lane i + 16*(k/4),
-// element k%4; B[k][j] in lane j + 16*(k/4), element k%4; D[i][j] in lane
+// Operands: lane l holds row (A) or column (B) l%16 and the 8 k values
+// 8*(l/16) .. 8*(l/16)+7 of the 32-deep step (A and B use the same k
+// assignment, so each product pairs matching k); D[i][j] in lane
 // j + 16*(i/4), element i%4.
 template <int BN, class LA, class LB, class EP>
 __global__ __launch_bounds__(kThreads) void gemm_kernel(int64_t M, int64_t N,
@@ -400,14 +404,14 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(int64_t M, int64_t N,
     }
     __syncthreads();
 #pragma unroll
-    for (int ks = 0; ks < kBK; ks += 16) {
-      const s4 a = *reinterpret_cast<const s4*>(
-          &As[16 * w + (lane & 15)][ks + 4 * (lane >> 4)]);
+    for (int ks = 0; ks < kBK; ks += 32) {
+      const bf8 a = *reinterpret_cast<const bf8*>(
+          &As[16 * w + (lane & 15)][ks + 8 * (lane >> 4)]);
 #pragma unroll
       for (int j = 0; j < BN / 16; ++j) {
-        const s4 b = *reinterpret_cast<const s4*>(
-            &Bs[16 * j + (lane & 15)][ks + 4 * (lane >> 4)]);
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        const bf8 b = *reinterpret_cast<const bf8*>(
+            &Bs[16 * j + (lane & 15)][ks + 8 * (lane >> 4)]);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
       }
     }
     __syncthreads();
