@@ -35,6 +35,7 @@
 // step kernels' block size and blocks per CU by timing iterations of each
 // candidate on the batch itself, cached per batch signature.
 #include <algorithm>
+#include <cstddef>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -79,8 +80,22 @@ struct BatchImg {
   int tw_log;   // Lloyd tile shape (Geometry::tw_log), the same in a batch
 };
 
+// One chunk of a bounded iteration's work: tiles [s, s + tiles) (global
+// tile numbers) of image img of the group.
+struct ChunkEnt {
+  int img;
+  int tiles;
+  int64_t s;
+};
+struct ChunkTable {
+  int n;     // entries this iteration
+  int pad[63];
+  ChunkEnt e[1];  // kMaxChunks entries follow the header
+};
+
 struct BatchArgs {
   BatchImg* img;
+  ChunkTable* chunks;  // per bounded iteration, built by batch_chunks
   int count;
   int64_t tiles;   // total Lloyd tiles
   int64_t ctiles;  // total CC tiles
@@ -418,15 +433,23 @@ __global__ __launch_bounds__(1024) void batch_step_plain(BatchArgs a, Group g) {
 // chunks balance the waves better, each chunk end costs a partial queue
 // drain and an image flush (batch_256: 224 / 218 / 202 / 207 / 233 ms and
 // the warm-start set: 121 / 109 / 107-111 / 108 / 111 ms at 64 / 96 / 128 /
-// 192 / 256 tiles).
+// 192 / 256 tiles while every chunk was 128 tiles).  With few running
+// images (the tail of a batch, or a batch of small images) 128-tile chunks
+// leave most waves idle, so each iteration's chunks are sized to the tiles
+// still running: about one chunk per wave, between kMinChunkTiles and
+// kBalanceTiles tiles.
 constexpr int64_t kBalanceTiles = 128;
+constexpr int64_t kMinChunkTiles = 16;
+constexpr int kMaxChunks = 32768;  // chunk table entries
 
 // The tiles [s, s1) of a group (possibly spanning images) for one wave.
 template <bool FIRST>
 __device__ __forceinline__ void bounded_run(const BatchArgs& a, const Group& g, WaveLds& w,
-                            int lane, int64_t s, int64_t s1) {
+                            int lane, int64_t s, int64_t s1, int known = -1) {
   while (s < s1) {
-    const int i = find_image<&BatchImg::tile0>(g.img, g.count, s);
+    const int i =
+        known >= 0 ? known : find_image<&BatchImg::tile0>(g.img, g.count, s);
+    known = -1;
     BatchImg& m = g.img[i];
     TileArgs ta{};
     static_cast<Geometry&>(ta) = image_geometry(m);
@@ -489,13 +512,91 @@ __global__ __launch_bounds__(512, 4) void batch_step_bounded(BatchArgs a, Group 
     bounded_run<FIRST>(a, g, w, lane, g.t0 + n * gw / W,
                        g.t0 + n * (gw + 1) / W);
   } else {
-    // failing tiles come in patches: chunks of kBalanceTiles dealt out
-    // round-robin, so that every wave sees every part of the images
-    for (int64_t c = g.t0 + gw * kBalanceTiles; c < g.t1;
-         c += W * kBalanceTiles)
-      bounded_run<FIRST>(a, g, w, lane, c,
-                         std::min<int64_t>(g.t1, c + kBalanceTiles));
+    // failing tiles come in patches: the running images' chunks (the table
+    // batch_chunks built for this iteration) are dealt out round-robin, so
+    // that every wave sees every part of the images
+    const int nc = a.chunks->n;
+    if (nc >= 0) {
+      for (int64_t c = gw; c < nc; c += W) {
+        const ChunkEnt e = a.chunks->e[c];
+        bounded_run<FIRST>(a, g, w, lane, e.s, e.s + e.tiles, e.img);
+      }
+    } else {  // no table (too many images): fixed chunks over every tile
+      for (int64_t c = g.t0 + gw * kBalanceTiles; c < g.t1;
+           c += W * kBalanceTiles)
+        bounded_run<FIRST>(a, g, w, lane, c,
+                           std::min<int64_t>(g.t1, c + kBalanceTiles));
+    }
   }
+}
+
+// The chunk table of one bounded iteration (one block): the group's running
+// images (not done), their tiles cut into chunks of
+// clamp(ceil(running tiles / target), kMinChunkTiles, kBalanceTiles) tiles
+// (more if the table would overflow), entries in image order.
+__global__ __launch_bounds__(1024) void batch_chunks(BatchArgs a, Group g,
+                                                     int target) {
+  __shared__ long long red[1024];
+  __shared__ long long cs_sh;
+  __shared__ int run_sh;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (g.count > kMaxChunks / 2) {  // the table could overflow: no table
+    if (tid == 0) a.chunks->n = -1;
+    return;
+  }
+  auto img_tiles = [&](int i) -> int64_t {
+    const BatchImg& m = g.img[i];
+    return m.done ? 0 : (int64_t)m.tiles_x * tiles_down(m.H, m.tw_log);
+  };
+  long long tot = 0;
+  for (int i = tid; i < g.count; i += nt) tot += img_tiles(i);
+  red[tid] = tot;
+  __syncthreads();
+  for (int o = nt / 2; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const long long T = red[0];
+    long long cs = (T + target - 1) / target;
+    cs = cs < kMinChunkTiles ? kMinChunkTiles : cs > kBalanceTiles ? kBalanceTiles : cs;
+    // entries <= T / cs + images <= kMaxChunks (images <= kMaxChunks / 2)
+    const long long room = kMaxChunks - g.count;
+    if (T / cs > room) cs = (T + room - 1) / room;
+    cs_sh = cs;
+    run_sh = 0;
+  }
+  __syncthreads();
+  const long long cs = cs_sh;
+  // images in rounds of nt: chunk counts, block prefix, entries
+  for (int base = 0; base < g.count; base += nt) {
+    const int i = base + tid;
+    const int64_t ti = i < g.count ? img_tiles(i) : 0;
+    const long long ci = (ti + cs - 1) / cs;
+    red[tid] = ci;
+    __syncthreads();
+    for (int o = 1; o < nt; o <<= 1) {
+      const long long add = tid >= o ? red[tid - o] : 0;
+      __syncthreads();
+      red[tid] += add;
+      __syncthreads();
+    }
+    const long long first = run_sh + red[tid] - ci;  // exclusive
+    if (ti > 0) {
+      const int64_t t0 = g.img[i].tile0;
+      for (long long j = 0; j < ci; ++j) {
+        ChunkEnt e;
+        e.img = i;
+        e.s = t0 + j * cs;
+        e.tiles = (int)std::min<int64_t>(cs, ti - j * cs);
+        a.chunks->e[first + j] = e;
+      }
+    }
+    __syncthreads();
+    if (tid == nt - 1) run_sh += (int)red[tid];
+    __syncthreads();
+  }
+  if (tid == 0) a.chunks->n = run_sh;
 }
 
 // Labels of the plain images: the assignment of the last iteration, i.e.
@@ -710,6 +811,7 @@ __global__ __launch_bounds__(BatchCcTile::kThreads) void batch_cc_final(
 // ------------------------------------------------------------------ host ---
 
 struct BatchPlan {
+  size_t chunks_off = 0;  // chunk table offset in temp
   std::vector<BatchImg> img;  // table order: plain images, then bounded ones
   std::vector<int> order;     // table entry -> caller's index
   int nplain = 0;
@@ -743,6 +845,8 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
   P->img.assign(count, BatchImg{});
   char* t = static_cast<char*>(temp);
   size_t o = align_up((size_t)count * sizeof(BatchImg), 256);
+  P->chunks_off = o;
+  o += align_up(offsetof(ChunkTable, e) + (size_t)kMaxChunks * sizeof(ChunkEnt), 256);
   int64_t tiles = 0, ctiles = 0, px = 0;
   for (int j = 0; j < count; ++j) {
     BatchImg& m = P->img[j];
@@ -820,9 +924,14 @@ static void launch_step(const BatchArgs& a, const Group& g, bool bounded,
   else if (first)
     hipLaunchKernelGGL(batch_step_bounded<true>, dim3(grid), dim3(t.threads),
                        lds, stream, a, g);
-  else
+  else {
+    // about one chunk per wave of the step grid (never below
+    // kMinChunkTiles, never above kBalanceTiles tiles)
+    hipLaunchKernelGGL(batch_chunks, dim3(1), dim3(1024), 0, stream, a, g,
+                       grid * nwaves);
     hipLaunchKernelGGL(batch_step_bounded<false>, dim3(grid), dim3(t.threads),
                        lds, stream, a, g);
+  }
 }
 
 // Autotuning of the Lloyd tile shape and the step kernels' block shapes,
@@ -995,6 +1104,7 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
                 temp_bytes, P.total_bytes);
   BatchArgs a;
   a.img = static_cast<BatchImg*>(temp);
+  a.chunks = reinterpret_cast<ChunkTable*>(static_cast<char*>(temp) + P.chunks_off);
   a.count = count;
   a.max_iter = std::max(0, max_iter);
   a.threshold = threshold;

@@ -58,8 +58,8 @@ def test_host_only_entry_points_without_gpu():
     ws = lib.split_hip_segment_batch_workspace(descs, 3)
     n = 2048 * 2048 + 37 * 53 + 1
     # state 4 B/px + local-root, linked-root and prefix words (3 x 1/8 B/px)
-    # + the CC tiles' edge columns (1/4 B/px)
-    assert 4 * n + 5 * n // 8 <= ws < 4 * n + 5 * n // 8 + (1 << 16)
+    # + the CC tiles' edge columns (1/4 B/px) + the chunk table (512 KiB)
+    assert 4 * n + 5 * n // 8 + (1 << 19) <= ws < 4 * n + 5 * n // 8 + (1 << 19) + (1 << 16)
     descs[1].k = 65
     assert lib.split_hip_segment_batch_workspace(descs, 3) == 0
     descs[1].k = 4
