@@ -536,10 +536,11 @@ __global__ __launch_bounds__(512, 4) void batch_step_bounded(BatchArgs a, Group 
 // (more if the table would overflow), entries in image order.
 __global__ __launch_bounds__(1024) void batch_chunks(BatchArgs a, Group g,
                                                      int target) {
-  __shared__ long long red[1024];
+  __shared__ long long wsum[1024 / kWave];
   __shared__ long long cs_sh;
-  __shared__ int run_sh;
+  __shared__ long long run_sh;
   const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & (kWave - 1), wv = tid / kWave, nw = nt / kWave;
   if (g.count > kMaxChunks / 2) {  // the table could overflow: no table
     if (tid == 0) a.chunks->n = -1;
     return;
@@ -548,16 +549,23 @@ __global__ __launch_bounds__(1024) void batch_chunks(BatchArgs a, Group g,
     const BatchImg& m = g.img[i];
     return m.done ? 0 : (int64_t)m.tiles_x * tiles_down(m.H, m.tw_log);
   };
+  // inclusive wave scan by shuffles
+  auto wave_scan = [&](long long x) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const long long y = __shfl_up(x, o);
+      x += lane >= o ? y : 0;
+    }
+    return x;
+  };
   long long tot = 0;
   for (int i = tid; i < g.count; i += nt) tot += img_tiles(i);
-  red[tid] = tot;
+  tot = wave_scan(tot);
+  if (lane == kWave - 1) wsum[wv] = tot;
   __syncthreads();
-  for (int o = nt / 2; o > 0; o >>= 1) {
-    if (tid < o) red[tid] += red[tid + o];
-    __syncthreads();
-  }
   if (tid == 0) {
-    const long long T = red[0];
+    long long T = 0;
+    for (int w = 0; w < nw; ++w) T += wsum[w];
     long long cs = (T + target - 1) / target;
     cs = cs < kMinChunkTiles ? kMinChunkTiles : cs > kBalanceTiles ? kBalanceTiles : cs;
     // entries <= T / cs + images <= kMaxChunks (images <= kMaxChunks / 2)
@@ -568,20 +576,17 @@ __global__ __launch_bounds__(1024) void batch_chunks(BatchArgs a, Group g,
   }
   __syncthreads();
   const long long cs = cs_sh;
-  // images in rounds of nt: chunk counts, block prefix, entries
+  // images in rounds of nt: chunk counts, their exclusive prefix, entries
   for (int base = 0; base < g.count; base += nt) {
     const int i = base + tid;
     const int64_t ti = i < g.count ? img_tiles(i) : 0;
     const long long ci = (ti + cs - 1) / cs;
-    red[tid] = ci;
+    const long long inc = wave_scan(ci);
+    if (lane == kWave - 1) wsum[wv] = inc;
     __syncthreads();
-    for (int o = 1; o < nt; o <<= 1) {
-      const long long add = tid >= o ? red[tid - o] : 0;
-      __syncthreads();
-      red[tid] += add;
-      __syncthreads();
-    }
-    const long long first = run_sh + red[tid] - ci;  // exclusive
+    long long before = run_sh;
+    for (int w = 0; w < wv; ++w) before += wsum[w];
+    const long long first = before + inc - ci;
     if (ti > 0) {
       const int64_t t0 = g.img[i].tile0;
       for (long long j = 0; j < ci; ++j) {
@@ -593,10 +598,11 @@ __global__ __launch_bounds__(1024) void batch_chunks(BatchArgs a, Group g,
       }
     }
     __syncthreads();
-    if (tid == nt - 1) run_sh += (int)red[tid];
+    if (tid == 0)
+      for (int w = 0; w < nw; ++w) run_sh += wsum[w];
     __syncthreads();
   }
-  if (tid == 0) a.chunks->n = run_sh;
+  if (tid == 0) a.chunks->n = (int)run_sh;
 }
 
 // Labels of the plain images: the assignment of the last iteration, i.e.
