@@ -292,8 +292,12 @@ struct BoundedWalker {
   }
 
   // Skip test of one tile; re-assigns it whole or queues its failing pixels.
-  __device__ __forceinline__ void visit(const TileGeo& g,
+  // The geometry is re-read as wave-uniform values, so that the interior
+  // test is a scalar branch (the compiler kept it in vector registers).
+  __device__ __forceinline__ void visit(const TileGeo& gv,
                                         const uint32_t (&ps)[4]) {
+    const TileGeo g{readfirstlane_i(gv.off), readfirstlane_i(gv.row0),
+                    readfirstlane_i(gv.col0)};
     const int col = g.col0 + lane_col(a, lane);
     int old[4];
     bool fail[4];
