@@ -609,7 +609,10 @@ int cluster_async(const float* pts, int64_t n, int width, int height,
   // Enqueue iterations in chunks; after queuing chunk c, wait for chunk c-1
   // and stop once its last block has reported convergence.  The GPU always
   // has the next chunk queued, and kernels past convergence exit at once.
-  constexpr int kChunk = 4;
+  // A run that cannot stop early (threshold < 0) is enqueued at once, small
+  // point sets in chunks of 16 (as the tiled path).
+  const int kChunk = threshold < 0.f ? std::max(max_iter, 1)
+                     : n < ((int64_t)1 << 22) ? 16 : 4;
   int launched = 0, chunk = 0;
   while (launched < max_iter) {
     const int cnt = std::min(kChunk, max_iter - launched);

@@ -881,8 +881,14 @@ int cluster_tiles_async(const float* pts, int64_t n, int width,
       tiles_launch_mode<false, kTile>(a, s, grid, threads, lds, stream);
   };
 
-  constexpr int kChunk = 4;
+  // Launches are enqueued in chunks, two chunks in flight, and the host
+  // stops enqueueing once the mailbox says done.  A run that cannot stop
+  // early (threshold < 0) is enqueued at once, small images in chunks of
+  // 16 (their launches take ~15 us); launches after the stop return at
+  // once.
   const int total = max_iter > 0 ? max_iter + 1 : 0;
+  const int kChunk = threshold < 0.f ? std::max(total, 1)
+                     : n < ((int64_t)1 << 22) ? 16 : 4;
   int launched = 0, chunk = 0;
   while (launched < total) {
     const int cnt = std::min(kChunk, total - launched);
