@@ -29,6 +29,9 @@ struct TileArgs : Geometry {
   int64_t nunits, unit_mul;
   // a tile with more failing pixels than this is re-assigned whole
   int whole_fails;
+  // neighbour lists of the ball re-assignment, two buffers of k lists
+  // (launch s writes buffer s % 2); null: box-pruned re-assignment
+  uint32_t* nb = nullptr;
 };
 
 // Per-pixel state: label | budget, where budget = slack + F[label] at the
@@ -75,14 +78,16 @@ __device__ __forceinline__ float bound_step(float f, double d2j, int j,
 constexpr int kAtomicChanges = 16;
 
 // The second half of reassign: given the pixels' colours, new labels and
-// slack, writes their state and accumulates.
-template <bool FULL>
+// slack, writes their state and accumulates (SHARED: into a slice that the
+// block's waves share).
+template <bool FULL, bool SHARED = false>
 __device__ __forceinline__ void commit_assignment(
     const TileArgs& a, const float* fl, QScale qs, int lane,
     const int64_t (&idx)[4], const bool (&v)[4], bool lvec,
     const int (&old)[4], const float (&x)[4], const float (&y)[4],
     const float (&z)[4], const int (&lab)[4], const float (&slack)[4],
     long long* slice, IRun& run) {
+  static_assert(!(FULL && SHARED), "the full iteration uses per-wave slices");
   const int k = a.k;
   uint32_t ns[4];
 #pragma unroll
@@ -135,8 +140,8 @@ __device__ __forceinline__ void commit_assignment(
     }
     if (nchg == 0) return;
     if (nchg <= kAtomicChanges) {
-      // few changes: each changed pixel adds into the wave's slice with LDS
-      // atomics (the slice is the wave's own)
+      // few changes: each changed pixel adds into the slice with LDS
+      // atomics
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         if (cb[i] == 0) continue;
@@ -156,8 +161,8 @@ __device__ __forceinline__ void commit_assignment(
         }
       }
     } else {
-      add_by_label(slice, k, lane, to, qx, qy, qz, 1);
-      add_by_label(slice, k, lane, from, qx, qy, qz, -1);
+      add_by_label<SHARED>(slice, k, lane, to, qx, qy, qz, 1);
+      add_by_label<SHARED>(slice, k, lane, from, qx, qy, qz, -1);
     }
   }
 }

@@ -28,15 +28,26 @@ __device__ __forceinline__ long long wave_sum_wide(int s) {
   return ((long long)wave_sum_i32(hi) << 16) + (long long)wave_sum_i32(lo);
 }
 
+// SHARED: the slice is shared by the waves of a block (LDS atomics); else it
+// is the wave's own.
+template <bool SHARED = false>
 __device__ __forceinline__ void slice_add64(long long* slice, int k, int label,
                                             long long sx, long long sy,
                                             long long sz, long long n,
                                             int lane) {
   if (lane == 0) {
-    slice[label] += sx;
-    slice[k + label] += sy;
-    slice[2 * k + label] += sz;
-    slice[3 * k + label] += n;
+    if constexpr (SHARED) {
+      auto* u = reinterpret_cast<unsigned long long*>(slice);
+      atomicAdd(u + label, (unsigned long long)sx);
+      atomicAdd(u + k + label, (unsigned long long)sy);
+      atomicAdd(u + 2 * k + label, (unsigned long long)sz);
+      atomicAdd(u + 3 * k + label, (unsigned long long)n);
+    } else {
+      slice[label] += sx;
+      slice[k + label] += sy;
+      slice[2 * k + label] += sz;
+      slice[3 * k + label] += n;
+    }
   }
 }
 
@@ -66,6 +77,7 @@ __device__ __forceinline__ void irun_flush(IRun& r, long long* slice, int k,
 // Adds sign * (q, 1) of each pixel with lab[i] >= 0 into its label's slice
 // entries, one distinct label at a time (|per-lane sum| <= 4 (2^23 - 1), so
 // a wave sum stays below 2^31).
+template <bool SHARED = false>
 __device__ __forceinline__ void add_by_label(long long* slice, int k, int lane,
                                              const int (&lab)[4],
                                              const int (&qx)[4],
@@ -95,7 +107,8 @@ __device__ __forceinline__ void add_by_label(long long* slice, int k, int lane,
     }
     const long long tx = wave_sum_i32(sx), ty = wave_sum_i32(sy),
                     tz = wave_sum_i32(sz), tn = wave_sum_i32(cnt);
-    slice_add64(slice, k, L, sign * tx, sign * ty, sign * tz, sign * tn, lane);
+    slice_add64<SHARED>(slice, k, L, sign * tx, sign * ty, sign * tz,
+                        sign * tn, lane);
   }
 }
 

@@ -32,6 +32,7 @@
 #include "lloyd_device.hpp"
 #include "lloyd_sums.hpp"
 #include "lloyd_bounded.hpp"
+#include "lloyd_ball.hpp"
 
 namespace splithip {
 
@@ -56,7 +57,9 @@ __device__ __forceinline__ long long* acc_buf(const TileArgs& a, int j) {
 
 // Finalisation of step s - 1 at the start of launch s >= 2 (see above).
 // LDS scratch: tot (4k int64), d2 (k doubles), cs (3k floats: c_s on
-// return); fs (k floats) receives F_s.  Returns the stop decision (the same
+// return); fs (k floats) receives F_s, and corr (if not null) the
+// neighbour-list correction |c_s[j] - c_{s-1}[j]| + max_i |c_s[i] -
+// c_{s-1}[i]|, rounded up.  Returns the stop decision (the same
 // in every block), also when an earlier launch already stopped (done_sh:
 // an LDS flag).  Every global load of the finalisation is issued before the
 // first barrier, so their latencies overlap (one round trip instead of
@@ -66,7 +69,8 @@ __device__ __forceinline__ bool tiles_prologue(const TileArgs& a, int s,
                                                long long* tot, double* d2,
                                                float* cs, float* fs,
                                                int* done_sh, QScale& qs,
-                                               float& maxabs) {
+                                               float& maxabs,
+                                               float* corr = nullptr) {
   const int k = a.k, V = 4 * k;
   const int tid = threadIdx.x;
   LloydState* st = a.state;
@@ -156,8 +160,11 @@ __device__ __forceinline__ bool tiles_prologue(const TileArgs& a, int s,
     }
   }
   __syncthreads();
-  if (tid < k)
+  if (tid < k) {
     fs[tid] = bound_step(f_old, d2[tid], tid, top[0], top[1], top_arg);
+    if (corr)
+      corr[tid] = to_float_up((sqrt(d2[tid]) + sqrt(top[0])) * (1.0 + 1e-9));
+  }
   const float shift = shift_sh;
   const int it = s - 1;
   const bool done = !(shift > threshold) || it >= max_iter;
@@ -214,7 +221,10 @@ __device__ __forceinline__ void list_units(const TileArgs& a, int64_t q0,
   }
 }
 
-template <int CPL, int MODE>
+// BALL: failing pixels are re-assigned by neighbour-ball walks
+// (assign_ball) instead of the box-pruned assign_segment, and the waves
+// share one slice.
+template <int CPL, int MODE, bool BALL = false>
 struct BoundedWalker {
   const TileArgs& a;
   const float (&cx)[CPL];
@@ -230,6 +240,31 @@ struct BoundedWalker {
   int lane, lane_off;  // lane l: row lane_row, columns lane_col..+3 of a tile
   IRun run;
   int qlen;            // wave-uniform
+  // BALL: centroids {x, y, z, -}, neighbour lists (nbm entries each) and
+  // their correction, all in LDS
+  const float4* cent4 = nullptr;
+  const uint32_t* nbl = nullptr;
+  int nbm = 0;
+  const float* corr = nullptr;
+
+  // Re-assigns up to 256 pixels (idx/v, previous labels old).
+  __device__ __forceinline__ void assign(const int64_t (&idx)[4],
+                                         const bool (&v)[4], bool lvec,
+                                         const int (&old)[4]) {
+    if constexpr (BALL) {
+      float x[4], y[4], z[4];
+      load_colours(a, idx, v, lvec, x, y, z);
+      int lab[4];
+      float slack[4];
+      assign_ball(cent4, nbl, nbm, corr, margin, a.k, x, y, z, v, old, lab,
+                  slack);
+      commit_assignment<false, true>(a, fl, qs, lane, idx, v, lvec, old, x,
+                                     y, z, lab, slack, slice, run);
+    } else {
+      reassign<CPL, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v, lvec,
+                           old, slice, run);
+    }
+  }
 
   __device__ __forceinline__ void tile_pixels(const TileGeo& g,
                                               int64_t (&idx)[4], bool (&v)[4],
@@ -278,8 +313,7 @@ struct BoundedWalker {
       idx[i] = qidx[v[i] ? e : 0];
       old[i] = v[i] ? qold[e] : -1;
     }
-    reassign<CPL, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v, false,
-                         old, slice, run);
+    assign(idx, v, false, old);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int e = 256 + lane + 64 * i;
@@ -329,8 +363,7 @@ struct BoundedWalker {
       tile_pixels(g, idx, v, lvec);
 #pragma unroll
       for (int i = 0; i < 4; ++i) old[i] = v[i] ? old[i] : -1;
-      reassign<CPL, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v, lvec,
-                           old, slice, run);
+      assign(idx, v, lvec, old);
       return;
     }
     // queue the failing pixels (in lane order within each slot)
@@ -432,13 +465,64 @@ struct BoundedWalker {
         idx[i] = v[i] ? q[e] : 0;
         old[i] = v[i] ? q[kQueueCap + e] : -1;
       }
-      reassign<CPL, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v, false,
-                           old, slice, run);
+      assign(idx, v, false, old);
     }
     qlen = 0;
   }
 };
 
+
+// The bounded walk of one block: its ranks [r0, r1) of the spread unit
+// order, listed kBlockUnits at a time, then the waves' queue remainders.
+template <class Walker>
+__device__ __forceinline__ void walk_block(const TileArgs& a, Walker& w,
+                                           int2* list, int* queues, int wave,
+                                           int nwaves) {
+  __shared__ unsigned unit_next;
+  const int64_t G = gridDim.x;
+  const int64_t r0 = a.nunits * blockIdx.x / G;
+  const int64_t r1 = a.nunits * (blockIdx.x + 1) / G;
+  for (int64_t q0 = r0; q0 < r1; q0 += kBlockUnits) {
+    const int cnt = (int)std::min<int64_t>(kBlockUnits, r1 - q0);
+    list_units(a, q0, cnt, list);
+    if (threadIdx.x == 0) unit_next = 0u;
+    __syncthreads();
+    w.walk(list, cnt, &unit_next);
+    __syncthreads();  // the list is refilled by the next round
+  }
+  w.finish_block(queues, wave, nwaves);
+}
+
+// The block's share of step s, summed over its nslices slices, is added
+// (device atomics) into copy blockIdx % kAccCopies of accumulator A[s % 3];
+// the next launch's prologue adds the copies into the totals.
+__device__ __forceinline__ void add_block_sums(const TileArgs& a, int s,
+                                               const long long* slices,
+                                               int nslices) {
+  const int V = 4 * a.k;
+  __syncthreads();
+  long long* acc = acc_buf(a, s) + (size_t)(blockIdx.x % kAccCopies) * V;
+  for (int vv = threadIdx.x; vv < V; vv += blockDim.x) {
+    long long tot = 0;
+    for (int w = 0; w < nslices; ++w) tot += slices[(size_t)w * V + vv];
+    if (tot != 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(acc + vv),
+                (unsigned long long)tot);
+  }
+}
+
+// Neighbour lists of centroids c (of launch s) for launch s + 1, when the
+// run uses them: list j by wave j / G of block j % G.
+template <class Cent>
+__device__ __forceinline__ void build_lists(const TileArgs& a, int s, Cent c,
+                                            int wave, int nwaves, int lane) {
+  if (a.nb == nullptr) return;
+  const int k = a.k, m = nb_len(k);
+  uint32_t* out = a.nb + (size_t)(s & 1) * k * m;
+  for (int j = wave * (int)gridDim.x + (int)blockIdx.x; j < k;
+       j += nwaves * (int)gridDim.x)
+    build_neighbours(c, k, j, m, out + (size_t)j * m, lane);
+}
 
 // Launch s of a tiled run (FULL: s == 1, every pixel assigned).
 // k <= 64: 1024-thread blocks at 4 waves per SIMD (<= 128 VGPRs).  Larger
@@ -465,10 +549,10 @@ __global__ __launch_bounds__(CPL >= 2 ? 256 : 1024, CPL >= 2 ? 2 : 1) void tiles
   const int V = 4 * k;
   long long* slices = lds64;                                   // nwaves x V
   float* fl = reinterpret_cast<float*>(lds64 + nwaves * V);    // F[0..k)
-  int* qidx = reinterpret_cast<int*>(fl + kMaxTileK) + wave * 2 * kQueueCap;
+  int* queues = reinterpret_cast<int*>(fl + kMaxTileK);
+  int* qidx = queues + wave * 2 * kQueueCap;
   int* qold = qidx + kQueueCap;
-  int2* list = reinterpret_cast<int2*>(
-      reinterpret_cast<int*>(fl + kMaxTileK) + nwaves * 2 * kQueueCap);
+  int2* list = reinterpret_cast<int2*>(queues + nwaves * 2 * kQueueCap);
   // this launch's centroids c_s (and F_s in fl): the initial ones for s = 1,
   // else finalised here from step s - 1 (scratch in the unit-list area,
   // 8 KiB + 2 KiB + 3 KiB at k = 256 < 16 KiB)
@@ -500,6 +584,7 @@ __global__ __launch_bounds__(CPL >= 2 ? 256 : 1024, CPL >= 2 ? 2 : 1) void tiles
     cz[t] = live ? cent[2 * k + j] : 0.f;
     cn[t] = fmaf(cz[t], cz[t], fmaf(cy[t], cy[t], cx[t] * cx[t]));
   }
+  if constexpr (FULL) build_lists(a, s, CentSoA{cent, k}, wave, nwaves, lane);
   __syncthreads();  // cent (LDS for s >= 2) is read; the list area is free
   long long* slice = slices + (size_t)wave * V;
 
@@ -512,36 +597,66 @@ __global__ __launch_bounds__(CPL >= 2 ? 256 : 1024, CPL >= 2 ? 2 : 1) void tiles
         a, TileRange{a.nseg * gw / W, a.nseg * (gw + 1) / W}, cx, cy, cz, cn,
         fl, qs, margin, slice, qidx, qold, lane);
   } else {
-    __shared__ unsigned unit_next;
     BoundedWalker<CPL, MODE> w{a,     cx,    cy,   cz,   cn,
                                fl,    qs,    margin, slice, qidx,
                                qold,  lane,  lane_row(a, lane) * a.width + lane_col(a, lane),
                                IRun{0, 0, 0, 0, -1, 0}, 0};
-    const int64_t G = gridDim.x;
-    const int64_t r0 = a.nunits * blockIdx.x / G;
-    const int64_t r1 = a.nunits * (blockIdx.x + 1) / G;
-    for (int64_t q0 = r0; q0 < r1; q0 += kBlockUnits) {
-      const int cnt = (int)std::min<int64_t>(kBlockUnits, r1 - q0);
-      list_units(a, q0, cnt, list);
-      if (threadIdx.x == 0) unit_next = 0u;
-      __syncthreads();
-      w.walk(list, cnt, &unit_next);
-      __syncthreads();  // the list is refilled by the next round
-    }
-    w.finish_block(reinterpret_cast<int*>(fl + kMaxTileK), wave, nwaves);
+    walk_block(a, w, list, queues, wave, nwaves);
   }
-  // the block's share of the step, summed over its waves, is added (device
-  // atomics) into copy blockIdx % kAccCopies of accumulator A[s % 3]; the
-  // next launch's prologue adds the copies into the totals
-  __syncthreads();
-  long long* acc = acc_buf(a, s) + (size_t)(blockIdx.x % kAccCopies) * V;
-  for (int vv = threadIdx.x; vv < V; vv += blockDim.x) {
-    long long tot = 0;
-    for (int w = 0; w < nwaves; ++w) tot += slices[(size_t)w * V + vv];
-    if (tot != 0)
-      atomicAdd(reinterpret_cast<unsigned long long*>(acc + vv),
-                (unsigned long long)tot);
+  add_block_sums(a, s, slices, nwaves);
+}
+
+// Bounded launch s >= 2 of a run with neighbour lists (k > 64): failing
+// pixels re-assigned by neighbour-ball walks.  1024-thread blocks, one per
+// CU; no centroids in registers, so the waves share one slice of the step
+// sums (LDS atomics) and the lists fit: at k = 256, 4 KiB of centroids,
+// 8 KiB slice, 32 KiB of lists, 64 KiB of queues, 16 KiB of units.
+template <int MODE>
+__global__ __launch_bounds__(1024, 1) void tiles_step_ball(TileArgs a, int s) {
+  extern __shared__ long long lds64[];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int nwaves = blockDim.x / kWave;
+  const int k = a.k, V = 4 * k, m = nb_len(k);
+  float4* cent4 = reinterpret_cast<float4*>(lds64);                    // kMaxTileK
+  long long* slice = reinterpret_cast<long long*>(cent4 + kMaxTileK);  // V
+  float* fl = reinterpret_cast<float*>(slice + V);                     // kMaxTileK
+  float* corr = fl + kMaxTileK;                                        // kMaxTileK
+  uint32_t* nbl = reinterpret_cast<uint32_t*>(corr + kMaxTileK);       // k m, even
+  int* queues = reinterpret_cast<int*>(nbl + ((k * m + 1) & ~1));
+  int* qidx = queues + wave * 2 * kQueueCap;
+  int* qold = qidx + kQueueCap;
+  int2* list = reinterpret_cast<int2*>(queues + nwaves * 2 * kQueueCap);
+  __shared__ int done_sh;
+  QScale qs;
+  float maxabs;
+  {
+    long long* tot = reinterpret_cast<long long*>(list);
+    double* d2 = reinterpret_cast<double*>(tot + 4 * kMaxTileK);
+    float* cs = reinterpret_cast<float*>(d2 + kMaxTileK);
+    if (tiles_prologue(a, s, tot, d2, cs, fl, &done_sh, qs, maxabs, corr))
+      return;
+    for (int j = threadIdx.x; j < k; j += blockDim.x)
+      cent4[j] = make_float4(cs[j], cs[k + j], cs[2 * k + j], 0.f);
   }
+  const uint32_t* nb_in = a.nb + (size_t)((s - 1) & 1) * k * m;
+  for (int i = threadIdx.x; i < k * m; i += blockDim.x) nbl[i] = nb_in[i];
+  for (int i = threadIdx.x; i < V; i += blockDim.x) slice[i] = 0;
+  __syncthreads();  // centroids, lists, correction and slice are set
+  build_lists(a, s, CentAoS{cent4}, wave, nwaves, lane);
+
+  const float margin = 1e-5f * (1.f + 4.f * maxabs);
+  const float none[1] = {0.f};
+  BoundedWalker<1, MODE, true> w{a,     none,  none, none,   none,
+                                 fl,    qs,    margin, slice, qidx,
+                                 qold,  lane,  lane_row(a, lane) * a.width + lane_col(a, lane),
+                                 IRun{0, 0, 0, 0, -1, 0}, 0};
+  w.cent4 = cent4;
+  w.nbl = nbl;
+  w.nbm = m;
+  w.corr = corr;
+  walk_block(a, w, list, queues, wave, nwaves);
+  add_block_sums(a, s, slice, 1);
 }
 
 // c_1 (the initial centroids) into cbuf[1], zero totals, accumulators and
@@ -688,6 +803,10 @@ __global__ void tiles_finish(const LloydState* st, const float* cbuf,
 // waves of a CU then share one pool of tiles in the bounded iterations and
 // finish together.
 static int tiles_threads(int k) { return k <= 64 ? 1024 : 256; }
+// Bounded iterations re-assign by neighbour-ball walks above this k (on
+// the 8192^2 k = 256 image a queued batch's colour box keeps ~230 of 256
+// centroids, its longest walk ~16).
+constexpr int kBallMinK = 65;
 // Step launches use at most this many blocks.
 static int tiles_max_blocks(int k) { return k <= 64 ? 512 : 1024; }
 // Units (pairs of adjacent tiles) of n pixels in rows of width.
@@ -707,6 +826,7 @@ struct TileLayout {
   unsigned* maxabs;
   uint32_t* pstate;
   F4* aos;
+  uint32_t* nb;
   size_t bytes;
 };
 
@@ -728,6 +848,8 @@ static TileLayout tiles_layout(void* temp, int64_t n, int k) {
   L.partials = L.totals ? L.totals + 2 * 4 * (size_t)k : nullptr;
   L.fcum = reinterpret_cast<float*>(take(2 * (size_t)k * sizeof(float)));
   L.maxabs = reinterpret_cast<unsigned*>(take(3 * sizeof(unsigned)));
+  L.nb = reinterpret_cast<uint32_t*>(
+      take(2 * (size_t)k * nb_len(k) * sizeof(uint32_t)));
   L.pstate = reinterpret_cast<uint32_t*>(
       take(((size_t)n + kStatePad) * sizeof(uint32_t)));
   L.aos = reinterpret_cast<F4*>(take((size_t)n * sizeof(F4)));
@@ -774,6 +896,11 @@ bool tiles_supported(int64_t n, int width, int k) {
 template <bool FULL, int MODE>
 static void tiles_launch_mode(const TileArgs& a, int s, int grid,
                               int threads, size_t lds, hipStream_t stream) {
+  if (!FULL && a.nb) {
+    hipLaunchKernelGGL(tiles_step_ball<MODE>, dim3(grid), dim3(threads), lds,
+                       stream, a, s);
+    return;
+  }
   switch (a.k <= 64 ? 1 : a.k <= 128 ? 2 : 4) {
     case 1:
       hipLaunchKernelGGL((tiles_step<1, MODE, FULL>), dim3(grid),
@@ -844,6 +971,7 @@ int cluster_tiles_async(const float* pts, int64_t n, int width,
   // and the 4096^2 k=32 one (77.7 / 73.4 / 71.0 / 70.1 us)
   a.whole_fails = kWholeTileFails;
   a.unit_mul = spread_multiplier(a.nunits);
+  a.nb = k >= kBallMinK ? L.nb : nullptr;
 
   const int threads = tiles_threads(k);
   const int nwaves = threads / kWave;
@@ -866,6 +994,17 @@ int cluster_tiles_async(const float* pts, int64_t n, int width,
   const int grid = (int)std::max<int64_t>(
       1, std::min<int64_t>({want, (int64_t)device_cu_count() * per_cu,
                             (int64_t)tiles_max_blocks(k)}));
+  // the ball kernel of the bounded launches: 1024 threads, one block per CU
+  const int ball_threads = 1024, ball_waves = ball_threads / kWave;
+  const size_t ball_lds =
+      kMaxTileK * sizeof(float4) + (size_t)4 * k * 8 +
+      2 * kMaxTileK * sizeof(float) +
+      (size_t)((k * nb_len(k) + 1) & ~1) * sizeof(uint32_t) +
+      (size_t)ball_waves * 2 * kQueueCap * sizeof(int) +
+      kBlockUnits * sizeof(int2);
+  const int ball_grid = (int)std::max<int64_t>(
+      1, std::min<int64_t>((a.nseg + ball_waves - 1) / ball_waves,
+                           (int64_t)device_cu_count()));
   // a lane's four pixels move as one dword-aligned 16-byte vector (any width)
   const bool vecmode = aligned4(pts) && aligned4(L.pstate);
   // launch s = 1 .. max_iter + 1: launch s finalises step s - 1 and runs
@@ -875,6 +1014,12 @@ int cluster_tiles_async(const float* pts, int64_t n, int width,
       tiles_launch_mode<true, kTileVec>(a, s, grid, threads, lds, stream);
     else if (s == 1)
       tiles_launch_mode<true, kTile>(a, s, grid, threads, lds, stream);
+    else if (a.nb && vecmode)
+      tiles_launch_mode<false, kTileVec>(a, s, ball_grid, ball_threads,
+                                         ball_lds, stream);
+    else if (a.nb)
+      tiles_launch_mode<false, kTile>(a, s, ball_grid, ball_threads, ball_lds,
+                                      stream);
     else if (vecmode)
       tiles_launch_mode<false, kTileVec>(a, s, grid, threads, lds, stream);
     else

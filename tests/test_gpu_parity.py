@@ -140,6 +140,50 @@ def test_cluster_image_bounded_iterations(torch_cuda, shape, k, iters):
         assert np.abs(c_last - want_c).max() <= CENT_TOL
 
 
+@pytest.mark.parametrize("case", ["packed", "duplicates", "noise"])
+def test_cluster_image_neighbour_walks(torch_cuda, case):
+    """Bounded iterations at k > 64 re-assign failing pixels by walking the
+    previous label's list of nearest centroids.  packed: 240 of 256 initial
+    centroids sit within 0.004 of one colour, so pixels far from them exhaust
+    the 32-entry lists and take the scan over every centroid; duplicates:
+    pairs of equal centroids (ties to the lower index); noise: uniform
+    colours, walks of every length.  The labels of an `iters` run must equal
+    the dense argmin against the centroids of an `iters - 1` run."""
+    torch = torch_cuda
+    w, h, iters = 300, 211, 8
+    rng = np.random.default_rng({"packed": 1, "duplicates": 2, "noise": 3}[case])
+    if case == "noise":
+        img = rng.random((3, w * h), dtype=np.float32)
+        k = 256
+        init = np.ascontiguousarray(img[:, rng.choice(w * h, k, replace=False)])
+    else:
+        spec = W.ImageSpec(w, h, seed=31, sites=40, shading=True, noise_sigma=0.02)
+        img = W.make_image(spec)
+        if case == "packed":
+            k = 256
+            init = np.empty((3, k), dtype=np.float32)
+            init[:, :240] = 0.5 + 0.004 * rng.random((3, 240), dtype=np.float32)
+            init[:, 240:] = img[:, rng.choice(w * h, 16, replace=False)]
+        else:
+            k = 96
+            half = img[:, rng.choice(w * h, k // 2, replace=False)]
+            init = np.ascontiguousarray(np.concatenate([half, half], axis=1))
+    say(f"cluster_image neighbour walks {case} {w}x{h} k={k} x{iters}")
+    P = dev(torch, img)
+    runs = {}
+    for n_it in (iters - 1, iters):
+        C = dev(torch, init)
+        L = torch.empty(w * h, dtype=torch.int32, device="cuda")
+        it, _ = sa.cluster(P, C, L, n_it, -1.0, width=w, height=h)
+        assert it == n_it
+        runs[n_it] = (host(C), host(L))
+    c_prev, _ = runs[iters - 1]
+    c_last, l_last = runs[iters]
+    assert np.array_equal(l_last, oracle.ref_label_points(c_prev, img))
+    want_mean, _ = oracle.centroids_f64(l_last, img, k)
+    assert np.abs(c_last - want_mean).max() <= CENT_TOL
+
+
 def test_cluster_image_repeatable(torch_cuda, plain):
     torch = torch_cuda
     w, img = plain
