@@ -116,7 +116,10 @@ def _counter_rows(exe, counter, workload, outdir):
     if not files:
         raise RuntimeError(f"no {name} counter file")
     with open(files[0], newline="") as f:
-        return [(r["Kernel_Name"], float(r["Counter_Value"])) for r in csv.DictReader(f)]
+        recs = list(csv.DictReader(f))
+    if recs and "Dispatch_Id" in recs[0]:  # in dispatch order
+        recs.sort(key=lambda r: int(r["Dispatch_Id"]))
+    return [(r["Kernel_Name"], float(r["Counter_Value"])) for r in recs]
 
 
 def measure_traffic(workload):
@@ -133,8 +136,19 @@ def measure_traffic(workload):
         for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU"):
             rows = _counter_rows(exe, counter, workload, Path(d) / counter.lower())
             copy = [v for k, v in rows if "copyBuffer" in k]
-            steps = [v for k, v in rows if k.startswith("void splithip::tiles_step")
-                     and "false>" in k]
+            # the bounded launches: tiles_step<CPL, MODE, false>; for k > 64
+            # each step is enqueued twice (box-pruned, then ball walk; one of
+            # them returns after the finalisation) and the pair is one step
+            box = [(k.startswith("void splithip::tiles_step<") and "false>" in k, v)
+                   for k, v in rows]
+            ball = [k.startswith("void splithip::tiles_step_ball<") for k, _ in rows]
+            steps = []
+            for i, (is_box, v) in enumerate(box):
+                if not is_box:
+                    continue
+                if i + 1 < len(rows) and ball[i + 1]:
+                    v += rows[i + 1][1]
+                steps.append(v)
             # launches 2 .. HI run bounded steps; launch HI + 1 only finalises
             want = ROOFLINE_HI
             if not copy or len(steps) != want:
@@ -832,10 +846,12 @@ def main():
         per_launch_ms = (t_hi - t_lo) / (hi - lo)
         bytes_per_launch = 12 * n
         achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
-        cpl = 1 if w.k <= 64 else 2 if w.k <= 128 else 4
+        kname = ("tiles_step<1, 3, false> (finalise the previous step, skip test, "
+                 "box-pruned re-assignment, exact sums)" if w.k <= 64 else
+                 "tiles_step_ball<3> (finalise the previous step, skip test, "
+                 "neighbour-ball re-assignment, exact sums)")
         out["roofline"] = {
-            "kernel": f"Lloyd iteration: tiles_step<{cpl}, 3, false> (finalise the previous "
-                      "step, skip test, re-assign, exact sums)",
+            "kernel": f"Lloyd iteration: {kname}",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

@@ -524,6 +524,36 @@ __device__ __forceinline__ void build_lists(const TileArgs& a, int s, Cent c,
     build_neighbours(c, k, j, m, out + (size_t)j * m, lane);
 }
 
+// Which of the two kernels enqueued for a bounded launch s of a run with
+// neighbour lists runs the step (the other returns after the
+// finalisation): the ball walk when the largest list correction is below
+// 0.6 of the mean nearest-neighbour distance in the lists, else the
+// box-pruned step.  A walk visits its label's list until the entries are
+// farther than the correction plus the pixel's distance to its centroid;
+// with corrections near the spacing of the centroids it reaches the end of
+// the list and rescans all k.  On the 8192^2 k = 256 image the ratio is
+// 1.74 / 0.80 / 0.46 / 0.34 in launches 2 / 3 / 4 / 5, where the box step
+// takes 2.9 / 2.8 / 2.6 / 2.6 ms and the walk 8.8 / 4.0 / 1.9 / 1.6 ms.
+// Both kernels compute it from the same data (corr from the finalisation,
+// the lists of launch s - 1), so exactly one of them runs.  Block-uniform;
+// red: 2 floats of LDS.
+__device__ __forceinline__ bool prefer_ball(const TileArgs& a, int s,
+                                            const float* corr, float* red) {
+  const int k = a.k, m = nb_len(k);
+  const uint32_t* nb = a.nb + (size_t)((s - 1) & 1) * k * m;
+  if (threadIdx.x < 2) red[threadIdx.x] = 0.f;
+  __syncthreads();
+  if (threadIdx.x < k) {
+    // non-negative floats order like their bits: integer atomics
+    atomicMax(reinterpret_cast<unsigned*>(red), __float_as_uint(corr[threadIdx.x]));
+    atomicAdd(red + 1, nb_dist(nb[(size_t)threadIdx.x * m]));
+  }
+  __syncthreads();
+  const bool ball = red[0] < 0.6f * (red[1] / (float)k);
+  __syncthreads();  // red is free again
+  return ball;
+}
+
 // Launch s of a tiled run (FULL: s == 1, every pixel assigned).
 // k <= 64: 1024-thread blocks at 4 waves per SIMD (<= 128 VGPRs).  Larger
 // k: 256-thread blocks at 2 waves per SIMD, whose 2 or 4 centroids per lane
@@ -568,7 +598,15 @@ __global__ __launch_bounds__(CPL >= 2 ? 256 : 1024, CPL >= 2 ? 2 : 1) void tiles
     long long* tot = reinterpret_cast<long long*>(list);
     double* d2 = reinterpret_cast<double*>(tot + 4 * kMaxTileK);
     float* cs = reinterpret_cast<float*>(d2 + kMaxTileK);
-    if (tiles_prologue(a, s, tot, d2, cs, fl, &done_sh, qs, maxabs)) return;
+    float* corr = cs + 3 * kMaxTileK;  // 14 KiB of the 16 KiB list area
+    if (tiles_prologue(a, s, tot, d2, cs, fl, &done_sh, qs, maxabs,
+                       a.nb ? corr : nullptr))
+      return;
+    if (a.nb) {
+      __shared__ float red[2];
+      __syncthreads();  // corr is written
+      if (prefer_ball(a, s, corr, red)) return;  // tiles_step_ball runs it
+    }
     cent = cs;
   }
   const float margin = 1e-5f * (1.f + 4.f * maxabs);
@@ -584,7 +622,8 @@ __global__ __launch_bounds__(CPL >= 2 ? 256 : 1024, CPL >= 2 ? 2 : 1) void tiles
     cz[t] = live ? cent[2 * k + j] : 0.f;
     cn[t] = fmaf(cz[t], cz[t], fmaf(cy[t], cy[t], cx[t] * cx[t]));
   }
-  if constexpr (FULL) build_lists(a, s, CentSoA{cent, k}, wave, nwaves, lane);
+  // lists for launch s + 1 (a run with neighbour lists)
+  build_lists(a, s, CentSoA{cent, k}, wave, nwaves, lane);
   __syncthreads();  // cent (LDS for s >= 2) is read; the list area is free
   long long* slice = slices + (size_t)wave * V;
 
@@ -636,6 +675,9 @@ __global__ __launch_bounds__(1024, 1) void tiles_step_ball(TileArgs a, int s) {
     float* cs = reinterpret_cast<float*>(d2 + kMaxTileK);
     if (tiles_prologue(a, s, tot, d2, cs, fl, &done_sh, qs, maxabs, corr))
       return;
+    __shared__ float red[2];
+    __syncthreads();  // corr is written
+    if (!prefer_ball(a, s, corr, red)) return;  // tiles_step ran it
     for (int j = threadIdx.x; j < k; j += blockDim.x)
       cent4[j] = make_float4(cs[j], cs[k + j], cs[2 * k + j], 0.f);
   }
@@ -895,8 +937,9 @@ bool tiles_supported(int64_t n, int width, int k) {
 
 template <bool FULL, int MODE>
 static void tiles_launch_mode(const TileArgs& a, int s, int grid,
-                              int threads, size_t lds, hipStream_t stream) {
-  if (!FULL && a.nb) {
+                              int threads, size_t lds, hipStream_t stream,
+                              bool ball = false) {
+  if (ball) {
     hipLaunchKernelGGL(tiles_step_ball<MODE>, dim3(grid), dim3(threads), lds,
                        stream, a, s);
     return;
@@ -1014,16 +1057,18 @@ int cluster_tiles_async(const float* pts, int64_t n, int width,
       tiles_launch_mode<true, kTileVec>(a, s, grid, threads, lds, stream);
     else if (s == 1)
       tiles_launch_mode<true, kTile>(a, s, grid, threads, lds, stream);
-    else if (a.nb && vecmode)
-      tiles_launch_mode<false, kTileVec>(a, s, ball_grid, ball_threads,
-                                         ball_lds, stream);
-    else if (a.nb)
-      tiles_launch_mode<false, kTile>(a, s, ball_grid, ball_threads, ball_lds,
-                                      stream);
     else if (vecmode)
       tiles_launch_mode<false, kTileVec>(a, s, grid, threads, lds, stream);
     else
       tiles_launch_mode<false, kTile>(a, s, grid, threads, lds, stream);
+    // with neighbour lists a bounded step is enqueued twice, box-pruned then
+    // ball walk; prefer_ball lets exactly one of them run it
+    if (s > 1 && a.nb && vecmode)
+      tiles_launch_mode<false, kTileVec>(a, s, ball_grid, ball_threads,
+                                         ball_lds, stream, true);
+    else if (s > 1 && a.nb)
+      tiles_launch_mode<false, kTile>(a, s, ball_grid, ball_threads, ball_lds,
+                                      stream, true);
   };
 
   // Launches are enqueued in chunks, two chunks in flight, and the host
