@@ -545,19 +545,28 @@ __device__ __forceinline__ void build_lists(const TileArgs& a, int s, Cent c,
 // the list and rescans all k.  On the 8192^2 k = 256 image the ratio is
 // 1.74 / 0.80 / 0.46 / 0.34 in launches 2 / 3 / 4 / 5, where the box step
 // takes 2.9 / 2.8 / 2.6 / 2.6 ms and the walk 8.8 / 4.0 / 1.9 / 1.6 ms.
-// Both kernels compute it from the same data (corr from the finalisation,
-// the lists of launch s - 1), so exactly one of them runs.  Block-uniform;
-// red: 2 floats of LDS.
+// Every block of both kernels computes it from the same data (corr from the
+// finalisation, the lists of launch s - 1) in the same order of operations
+// (wave 0: per-lane sums in index order, then a fixed DPP reduction; no
+// atomics, whose order would vary), so every block reaches the same
+// decision and exactly one kernel runs the step.  Block-uniform; red: 2
+// floats of LDS.
 __device__ __forceinline__ bool prefer_ball(const TileArgs& a, int s,
                                             const float* corr, float* red) {
   const int k = a.k, m = nb_len(k);
   const uint32_t* nb = a.nb + (size_t)((s - 1) & 1) * k * m;
-  if (threadIdx.x < 2) red[threadIdx.x] = 0.f;
-  __syncthreads();
-  if (threadIdx.x < k) {
-    // non-negative floats order like their bits: integer atomics
-    atomicMax(reinterpret_cast<unsigned*>(red), __float_as_uint(corr[threadIdx.x]));
-    atomicAdd(red + 1, nb_dist(nb[(size_t)threadIdx.x * m]));
+  if (threadIdx.x < kWave) {
+    float mx = 0.f, sum = 0.f;
+    for (int j = threadIdx.x; j < k; j += kWave) {
+      mx = fmaxf(mx, corr[j]);
+      sum += nb_dist(nb[(size_t)j * m]);
+    }
+    mx = wave_allreduce(mx, OpMax());
+    sum = wave_allreduce(sum, OpAdd());
+    if (threadIdx.x == 0) {
+      red[0] = mx;
+      red[1] = sum;
+    }
   }
   __syncthreads();
   const bool ball = red[0] < 0.6f * (red[1] / (float)k);
