@@ -123,25 +123,41 @@ __device__ __forceinline__ void assign_ball(
     lim[i] = corr[a] + __builtin_amdgcn_sqrtf(best[i]) * (1.f + 1e-6f) + margin;
     act[i] = v[i];
   }
+  // Branch-free steps: the four slots' list entries are read together (and
+  // the next step's one step ahead), then their four centroids, so that a
+  // step waits for about one LDS round trip instead of two per slot (per-slot
+  // branches serialised eight of them).  Inactive slots read in-bounds
+  // entries (a = 0 for invalid pixels) and discard them.
+  uint32_t e[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) e[i] = nb[base[i]];
   for (int p = 0; p < m; ++p) {
     if (ballot(act[0] || act[1] || act[2] || act[3]) == 0) break;
+    const int pn = p + 1 < m ? p + 1 : p;
+    uint32_t en[4];
+    int j[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      if (!act[i]) continue;
-      const uint32_t e = nb[base[i] + p];
-      const float lb = nb_dist(e) - lim[i];  // every centroid not yet visited
-      if (lb > 0.f && lb * lb >= second[i]) {
-        act[i] = false;
-      } else {
-        const int j = nb_index(e);
-        const float4 c = cent[j];
-        const float d = sqdist(x[i], y[i], z[i], c.x, c.y, c.z);
-        const bool better = d < best[i] || (d == best[i] && j < bk[i]);
-        // best <= second: the median is the new runner-up
-        second[i] = __builtin_amdgcn_fmed3f(best[i], d, second[i]);
-        best[i] = fminf(best[i], d);
-        bk[i] = better ? j : bk[i];
-      }
+      en[i] = nb[base[i] + pn];
+      const float lb = nb_dist(e[i]) - lim[i];  // every centroid not yet visited
+      act[i] = act[i] & !((lb > 0.f) & (lb * lb >= second[i]));
+      j[i] = nb_index(e[i]);
+    }
+    float4 c[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[i] = cent[j[i]];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float d = sqdist(x[i], y[i], z[i], c[i].x, c[i].y, c[i].z);
+      // bitwise, not short-circuit: no branches
+      const bool take =
+          act[i] & ((d < best[i]) | ((d == best[i]) & (j[i] < bk[i])));
+      // best <= second: the median is the new runner-up
+      second[i] = act[i] ? __builtin_amdgcn_fmed3f(best[i], d, second[i])
+                         : second[i];
+      bk[i] = take ? j[i] : bk[i];
+      best[i] = act[i] ? fminf(best[i], d) : best[i];
+      e[i] = en[i];
     }
   }
   // walks that used the whole list: the unlisted centroids are at least
