@@ -12,8 +12,10 @@
 //
 // Lists: launch s builds, from its centroids c_s, the m nearest other
 // centroids of every centroid (one wave per list), and launch s + 1 walks
-// them.  An entry holds a lower bound of |c_s[a] - c_s[j]| with its low 8
-// bits replaced by j, so that entries sort by distance, then index.  In
+// them.  An entry holds a lower bound of |c_s[a] - c_s[j]| (the top 20 bits
+// of its float: 11 mantissa bits, truncated) over 16 j, the byte offset of
+// centroid j in a float4 table, so that entries sort by distance, then
+// index, and a walk step gets the centroid's LDS address with one AND.  In
 // launch s + 1 the centroids have moved: |c_{s+1}[a] - c_{s+1}[j]| >=
 // entry - mv_a - mv_j >= entry - corr[a], corr[a] = mv_a + max_j mv_j.
 // For a pixel x of previous label a and every centroid j at or after list
@@ -37,9 +39,10 @@ __host__ __device__ __forceinline__ int nb_len(int k) {
   return k - 1 < kNbMax ? k - 1 : kNbMax;
 }
 __device__ __forceinline__ float nb_dist(uint32_t e) {
-  return __uint_as_float(e & 0xFFFFFF00u);
+  return __uint_as_float(e & 0xFFFFF000u);
 }
-__device__ __forceinline__ int nb_index(uint32_t e) { return (int)(e & 0xFFu); }
+// 16 j: the byte offset of centroid j in a float4 table
+__device__ __forceinline__ int nb_off(uint32_t e) { return (int)(e & 0xFF0u); }
 
 __device__ __forceinline__ unsigned wave_min_u32(unsigned x) {
   x = min(x, (unsigned)dpp_i<kDppQuadXor1>((int)x));
@@ -86,7 +89,7 @@ __device__ __forceinline__ void build_neighbours(Cent c, int k, int a, int m,
       const float d =
           __builtin_amdgcn_sqrtf(sqdist(ca.x, ca.y, ca.z, cj.x, cj.y, cj.z)) *
           (1.f - 1e-6f);
-      key[t] = (__float_as_uint(d) & 0xFFFFFF00u) | (uint32_t)j;
+      key[t] = (__float_as_uint(d) & 0xFFFFF000u) | ((uint32_t)j << 4);
     }
   }
   for (int r = 0; r < m; ++r) {
@@ -108,70 +111,86 @@ __device__ __forceinline__ void assign_ball(
     float margin, int k, const float (&x)[4], const float (&y)[4],
     const float (&z)[4], const bool (&v)[4], const int (&old)[4],
     int (&lab)[4], float (&slack)[4]) {
+  // bk: 16 x the index of the best centroid (its byte offset in cent; the
+  // offsets order like the indices)
+  const char* cb = reinterpret_cast<const char*>(cent);
   float best[4], second[4], lim[4];
   int bk[4], base[4];
-  bool act[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int a = v[i] ? old[i] : 0;
     const float4 c = cent[a];
     best[i] = sqdist(x[i], y[i], z[i], c.x, c.y, c.z);
     second[i] = INFINITY;
-    bk[i] = a;
+    bk[i] = 16 * a;
     base[i] = a * m;
-    // |x - c_a| <= the float distance (1 + 1e-6); margin covers the rest
-    lim[i] = corr[a] + __builtin_amdgcn_sqrtf(best[i]) * (1.f + 1e-6f) + margin;
-    act[i] = v[i];
+    // |x - c_a| <= the float distance (1 + 1e-6); margin covers the rest.
+    // Invalid pixels: -inf, so that their walk stops at once.
+    lim[i] = v[i] ? corr[a] + __builtin_amdgcn_sqrtf(best[i]) * (1.f + 1e-6f) +
+                        margin
+                  : -INFINITY;
   }
-  // Branch-free steps: the four slots' list entries are read together (and
-  // the next step's one step ahead), then their four centroids, so that a
-  // step waits for about one LDS round trip instead of two per slot (per-slot
-  // branches serialised eight of them).  Inactive slots read in-bounds
-  // entries (a = 0 for invalid pixels) and discard them.
+  // A walk stops at the first entry whose bound puts every centroid not yet
+  // visited at or beyond the runner-up: (lb > 0) & (lb^2 >= second).  The
+  // test is sticky without a flag: entries ascend and a stopped walk's
+  // runner-up no longer changes.  Branch-free steps: the four slots' list
+  // entries are read together (and the next step's one step ahead), then
+  // their four centroids, so that a step waits for about one LDS round trip
+  // instead of two per slot (per-slot branches serialised eight of them);
+  // stopped slots read in-bounds entries (a = 0 for invalid pixels) and
+  // count their centroid as infinitely far, which leaves best, second and
+  // bk unchanged.
+  auto stops = [&](uint32_t e, int i) {
+    const float lb = nb_dist(e) - lim[i];
+    return (lb > 0.f) & (lb * lb >= second[i]);
+  };
   uint32_t e[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) e[i] = nb[base[i]];
   for (int p = 0; p < m; ++p) {
-    if (ballot(act[0] || act[1] || act[2] || act[3]) == 0) break;
+    bool stop[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) stop[i] = stops(e[i], i);
+    if (ballot(!(stop[0] & stop[1] & stop[2] & stop[3])) == 0) break;
     const int pn = p + 1 < m ? p + 1 : p;
     uint32_t en[4];
-    int j[4];
+    float4 c[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       en[i] = nb[base[i] + pn];
-      const float lb = nb_dist(e[i]) - lim[i];  // every centroid not yet visited
-      act[i] = act[i] & !((lb > 0.f) & (lb * lb >= second[i]));
-      j[i] = nb_index(e[i]);
+      c[i] = *reinterpret_cast<const float4*>(cb + nb_off(e[i]));
     }
-    float4 c[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) c[i] = cent[j[i]];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const float d = sqdist(x[i], y[i], z[i], c[i].x, c[i].y, c[i].z);
+      const int off = nb_off(e[i]);
+      const float dd = sqdist(x[i], y[i], z[i], c[i].x, c[i].y, c[i].z);
+      const float d = stop[i] ? INFINITY : dd;
       // bitwise, not short-circuit: no branches
-      const bool take =
-          act[i] & ((d < best[i]) | ((d == best[i]) & (j[i] < bk[i])));
+      const bool take = (d < best[i]) | ((d == best[i]) & (off < bk[i]));
       // best <= second: the median is the new runner-up
-      second[i] = act[i] ? __builtin_amdgcn_fmed3f(best[i], d, second[i])
-                         : second[i];
-      bk[i] = take ? j[i] : bk[i];
-      best[i] = act[i] ? fminf(best[i], d) : best[i];
+      second[i] = __builtin_amdgcn_fmed3f(best[i], d, second[i]);
+      bk[i] = take ? off : bk[i];
+      best[i] = fminf(best[i], d);
       e[i] = en[i];
     }
   }
-  // walks that used the whole list: the unlisted centroids are at least
-  // the last entry's bound away
+  // The unlisted centroids (m < k - 1) are at least the last entry's bound
+  // away: a walk whose stop test holds at the last entry is settled (every
+  // walk that stopped early, by stickiness); else the bound still caps the
+  // runner-up if it exceeds the nearest, or the walk is redone over all k.
   bool redo[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     redo[i] = false;
-    if (act[i] && m < k - 1) {
-      const float lb = nb_dist(nb[base[i] + m - 1]) - lim[i];
-      if (lb > 0.f && lb * lb > best[i])
-        second[i] = fminf(second[i], lb * lb);
-      else
-        redo[i] = true;
+    if (m < k - 1) {
+      const uint32_t last = nb[base[i] + m - 1];
+      const float lb = nb_dist(last) - lim[i];
+      if (!stops(last, i)) {
+        if (lb > 0.f && lb * lb > best[i])
+          second[i] = fminf(second[i], lb * lb);
+        else
+          redo[i] = true;
+      }
     }
   }
   if (ballot(redo[0] || redo[1] || redo[2] || redo[3])) {
@@ -189,14 +208,14 @@ __device__ __forceinline__ void assign_ball(
         if (!redo[i]) continue;
         const float d = sqdist(x[i], y[i], z[i], c.x, c.y, c.z);
         second[i] = __builtin_amdgcn_fmed3f(best[i], d, second[i]);
-        bk[i] = d < best[i] ? j : bk[i];
+        bk[i] = d < best[i] ? 16 * j : bk[i];
         best[i] = fminf(best[i], d);
       }
     }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    lab[i] = v[i] ? bk[i] : -1;
+    lab[i] = v[i] ? bk[i] >> 4 : -1;
     // the same 2e-5 relative shrink as assign_segment's slack
     const float s = __builtin_amdgcn_sqrtf(second[i] * (1.f - 2e-5f)) -
                     __builtin_amdgcn_sqrtf(best[i] * (1.f + 2e-5f));
