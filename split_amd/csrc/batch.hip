@@ -61,6 +61,7 @@ struct BatchImg {
   int W, H, k, tiles_x;  // 16x16 Lloyd tiles
   int ctiles_x, pad0;    // 64x64 CC tiles
   int64_t tile0;         // first global Lloyd tile
+  int64_t cost0;         // the group's tile cost before this image
   int64_t ctile0;        // first global CC tile
   int64_t px0;           // first global pixel (multiple of 64)
   float* cbuf;           // 2 x 3k
@@ -110,7 +111,16 @@ struct Group {
   BatchImg* img;
   int count;
   int64_t t0, t1;
+  int64_t c0 = 0, c1 = 0;  // tile cost of the group: [c0, c1)
 };
+
+// Estimated cost of a Lloyd tile of an image with k centroids in the full
+// first iteration (the box keeps more candidates at larger k), in units of
+// ~7 VALU instructions: the first iteration splits the group's cost, not its
+// tiles, evenly over the waves.
+__host__ __device__ __forceinline__ int64_t tile_cost(int k) {
+  return 32 + k;
+}
 
 // Images with k >= kBoundedMinK take bounded iterations.  They pay for
 // reading and writing the per-pixel state and for queueing, and win at every
@@ -506,11 +516,18 @@ __global__ __launch_bounds__(512, 4) void batch_step_bounded(BatchArgs a, Group 
 
   const int64_t gw = (int64_t)blockIdx.x * nwaves + wave;
   const int64_t W = (int64_t)gridDim.x * nwaves;
-  const int64_t n = g.t1 - g.t0;
   if constexpr (FIRST) {
-    // every tile costs the same: one contiguous run per wave
-    bounded_run<FIRST>(a, g, w, lane, g.t0 + n * gw / W,
-                       g.t0 + n * (gw + 1) / W);
+    // one contiguous run per wave, of equal estimated cost: wave gw takes
+    // the tiles whose cost interval starts in its share of [c0, c1) (with
+    // equal tile counts the waves on k = 64 images set the launch's time)
+    const int64_t C = g.c1 - g.c0;
+    auto tile_at = [&](int64_t c) {
+      const int i = find_image<&BatchImg::cost0>(g.img, g.count, c);
+      const int64_t wt = tile_cost(g.img[i].k);
+      return g.img[i].tile0 + (c - g.img[i].cost0 + wt - 1) / wt;
+    };
+    bounded_run<FIRST>(a, g, w, lane, tile_at(g.c0 + C * gw / W),
+                       tile_at(g.c0 + C * (gw + 1) / W));
   } else {
     // failing tiles come in patches: the running images' chunks (the table
     // batch_chunks built for this iteration) are dealt out round-robin, so
@@ -824,6 +841,7 @@ struct BatchPlan {
   int64_t tiles_plain = 0;
   size_t total_bytes = 0;
   int64_t tiles = 0, ctiles = 0, px = 0;
+  int64_t cost_bounded = 0;  // tile cost of the bounded group
 };
 
 static int make_plan(const split_hip_image_desc* d, int count, void* temp,
@@ -853,9 +871,10 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
   size_t o = align_up((size_t)count * sizeof(BatchImg), 256);
   P->chunks_off = o;
   o += align_up(offsetof(ChunkTable, e) + (size_t)kMaxChunks * sizeof(ChunkEnt), 256);
-  int64_t tiles = 0, ctiles = 0, px = 0;
+  int64_t tiles = 0, ctiles = 0, px = 0, cost = 0;
   for (int j = 0; j < count; ++j) {
     BatchImg& m = P->img[j];
+    if (j == P->nplain) cost = 0;  // each group's costs count from 0
     const split_hip_image_desc& e = d[P->order[j]];
     const int64_t n = (int64_t)e.width * e.height;
     if (j == P->nplain) P->tiles_plain = tiles;
@@ -872,9 +891,12 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
     m.tiles_x = (int)tiles_across(e.width, tw_log);
     m.ctiles_x = (e.width + kCcSide - 1) / kCcSide;
     m.tile0 = tiles;
+    m.cost0 = cost;
     m.ctile0 = ctiles;
     m.px0 = px;
-    tiles += (int64_t)m.tiles_x * tiles_down(e.height, tw_log);
+    const int64_t img_tiles = (int64_t)m.tiles_x * tiles_down(e.height, tw_log);
+    tiles += img_tiles;
+    cost += img_tiles * tile_cost(e.k);
     const int64_t img_ctiles =
         (int64_t)m.ctiles_x * ((e.height + kCcSide - 1) / kCcSide);
     ctiles += img_ctiles;
@@ -895,6 +917,7 @@ static int make_plan(const split_hip_image_desc* d, int count, void* temp,
   }
   if (P->nplain == count) P->tiles_plain = tiles;
   P->tiles = tiles;
+  P->cost_bounded = P->nplain < count ? cost : 0;
   P->ctiles = ctiles;
   P->px = px;
   P->total_bytes = o;
@@ -1131,7 +1154,8 @@ int split_hip_segment_batch(const split_hip_image_desc* images, int32_t count,
                                    (size_t)count * sizeof(BatchImg),
                                    hipMemcpyHostToDevice, stream));
     G.plain = Group{a.img, P.nplain, 0, P.tiles_plain};
-    G.bounded = Group{a.img + P.nplain, count - P.nplain, P.tiles_plain, P.tiles};
+    G.bounded = Group{a.img + P.nplain, count - P.nplain, P.tiles_plain, P.tiles,
+                      0, P.cost_bounded};
     return SPLIT_HIP_OK;
   };
   const BatchTuneKey tkey{device, count, P.px, P.nplain, kp, kb};
