@@ -581,7 +581,11 @@ __device__ __forceinline__ bool prefer_ball(const TileArgs& a, int s,
 // on the 8192^2 k = 256 image 1,756 -> 1,723 us per iteration without).
 template <int CPL, int MODE, bool FULL>
 __global__ __launch_bounds__(CPL >= 2 ? 256 : 1024, CPL >= 2 ? 2 : 1) void tiles_step(TileArgs a, int s) {
-  extern __shared__ long long lds64[];
+  // 16-byte aligned: the float4 tables in it are read with ds_read_b128
+  // (the static LDS variables before it had left it 16-aligned by chance;
+  // four more bytes of them made every centroid read of the walks
+  // unaligned: 410 -> 762 us per 8192^2 k = 256 iteration)
+  extern __shared__ __align__(16) long long lds64[];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int nwaves = blockDim.x / kWave;
@@ -672,7 +676,11 @@ __global__ __launch_bounds__(CPL >= 2 ? 256 : 1024, CPL >= 2 ? 2 : 1) void tiles
 // 8 KiB slice, 32 KiB of lists, 64 KiB of queues, 16 KiB of units.
 template <int MODE>
 __global__ __launch_bounds__(1024, 1) void tiles_step_ball(TileArgs a, int s) {
-  extern __shared__ long long lds64[];
+  // 16-byte aligned: the float4 tables in it are read with ds_read_b128
+  // (the static LDS variables before it had left it 16-aligned by chance;
+  // four more bytes of them made every centroid read of the walks
+  // unaligned: 410 -> 762 us per 8192^2 k = 256 iteration)
+  extern __shared__ __align__(16) long long lds64[];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int nwaves = blockDim.x / kWave;
