@@ -676,6 +676,12 @@ __global__ __launch_bounds__(CPL >= 2 ? 256 : 1024, CPL >= 2 ? 2 : 1) void tiles
 // 8 KiB slice, 32 KiB of lists, 64 KiB of queues, 16 KiB of units.
 template <int MODE>
 __global__ __launch_bounds__(1024, 1) void tiles_step_ball(TileArgs a, int s) {
+  // Failing pixels are gathered from the three planes, not the interleaved
+  // copy: ~40% of a tile's pixels fail here, so the planes' lines are read
+  // anyway (8192^2 k = 256 iteration 405.8 -> 399.1 us; the box-pruned
+  // walker's sparser failures stay on the interleaved copy, one line per
+  // pixel: 4096^2 k = 32 63.4-64.1 against 64.6-65.2 us on the planes)
+  a.aos = nullptr;
   // 16-byte aligned: the float4 tables in it are read with ds_read_b128
   // (the static LDS variables before it had left it 16-aligned by chance;
   // four more bytes of them made every centroid read of the walks
