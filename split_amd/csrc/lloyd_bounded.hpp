@@ -151,7 +151,7 @@ __device__ __forceinline__ void commit_assignment(
 // Re-assigns up to 256 pixels (4 per lane: idx/v) whose previous labels are
 // old[] (-1: none), writes their new state, and accumulates: every pixel
 // (FULL) or the +/- changes of label (bounded iterations).
-template <int CPL, bool FULL>
+template <int CPL, bool FULL, bool PRUNE = true>
 __device__ __forceinline__ void reassign(
     const TileArgs& a, const float (&cx)[CPL], const float (&cy)[CPL],
     const float (&cz)[CPL], const float (&cn)[CPL], const float* fl,
@@ -161,7 +161,8 @@ __device__ __forceinline__ void reassign(
   load_colours(a, idx, v, lvec, x, y, z);
   int lab[4];
   float slack[4];
-  assign_segment<CPL, true>(cx, cy, cz, cn, a.k, lane, x, y, z, v, lab, slack);
+  assign_segment<CPL, true, PRUNE>(cx, cy, cz, cn, a.k, lane, x, y, z, v, lab,
+                                   slack);
   commit_assignment<FULL>(a, fl, qs, lane, idx, v, lvec, old, x, y, z,
                           lab, slack, slice, run);
 }
@@ -217,8 +218,9 @@ __device__ __forceinline__ void walk_tiles(
       idx[i] = qidx[v[i] ? e : 0];
       old[i] = v[i] ? qold[e] : -1;
     }
-    reassign<CPL, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v, false,
-                         old, slice, run);
+    // queued pixels come from all over the chunk: no box pruning
+    reassign<CPL, false, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v,
+                                false, old, slice, run);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int e = 256 + lane + 64 * i;

@@ -160,12 +160,46 @@ __device__ __forceinline__ float sqdist(float px, float py, float pz, float cx,
 // one (non-squared, rounded down): the second-best evaluated distance, the
 // box lower bound of every box-pruned centroid, and for bisector-pruned
 // centroids |p-c_j|^2 >= |p-c_g|^2 + 2 (b_j - max_box p.w_j) >= best^2 + 2 m.
-template <int CPL, bool TOP2>
+// PRUNE = false evaluates every centroid (batches whose pixels come from
+// all over an image, whose box keeps nearly all of them anyway).
+template <int CPL, bool TOP2, bool PRUNE = true>
 __device__ __forceinline__ void assign_segment(
     const float (&cx)[CPL], const float (&cy)[CPL], const float (&cz)[CPL],
     const float (&cn)[CPL], int k, int lane, const float (&x)[4],
     const float (&y)[4], const float (&z)[4], const bool (&v)[4],
     int (&lab)[4], float* slack) {
+  float bd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  float bd2[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  int bk[4] = {0, 0, 0, 0};
+  float lbox = INFINITY, mbis = INFINITY;  // per-lane, reduced below
+  // One candidate (lane bl of register t) against the four pixels.
+  auto eval = [&](int t, int bl) {
+    const float ccx = readlane_f(cx[t], bl);
+    const float ccy = readlane_f(cy[t], bl);
+    const float ccz = readlane_f(cz[t], bl);
+    const int j = 64 * t + bl;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float d = sqdist(x[i], y[i], z[i], ccx, ccy, ccz);
+      const bool better = d < bd[i];
+      // with bd <= bd2 always, the median of (bd, d, bd2) is the new
+      // runner-up: one v_med3_f32
+      if constexpr (TOP2) bd2[i] = __builtin_amdgcn_fmed3f(bd[i], d, bd2[i]);
+      bd[i] = fminf(bd[i], d);
+      bk[i] = better ? j : bk[i];
+    }
+  };
+  if constexpr (!PRUNE) {
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+      uint64_t m = ballot(lane + 64 * t < k);
+      while (m) {
+        const int bl = __builtin_ctzll(m);
+        m &= m - 1;
+        eval(t, bl);
+      }
+    }
+  } else {
   // Colour box of the segment's valid pixels.
   float lx = INFINITY, ly = INFINITY, lz = INFINITY;
   float hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
@@ -231,10 +265,6 @@ __device__ __forceinline__ void assign_segment(
   const float ax = fmaxf(fabsf(lx), fabsf(hx)), ay = fmaxf(fabsf(ly), fabsf(hy)),
               az = fmaxf(fabsf(lz), fabsf(hz));
 
-  float bd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-  float bd2[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-  int bk[4] = {0, 0, 0, 0};
-  float lbox = INFINITY, mbis = INFINITY;  // per-lane, reduced below
 #pragma unroll
   for (int t = 0; t < CPL; ++t) {
     const float wx = cx[t] - gx, wy = cy[t] - gy, wz = cz[t] - gz;
@@ -255,21 +285,9 @@ __device__ __forceinline__ void assign_segment(
     while (m) {
       const int bl = __builtin_ctzll(m);
       m &= m - 1;
-      const float ccx = readlane_f(cx[t], bl);
-      const float ccy = readlane_f(cy[t], bl);
-      const float ccz = readlane_f(cz[t], bl);
-      const int j = 64 * t + bl;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float d = sqdist(x[i], y[i], z[i], ccx, ccy, ccz);
-        const bool better = d < bd[i];
-        // with bd <= bd2 always, the median of (bd, d, bd2) is the new
-        // runner-up: one v_med3_f32
-        if constexpr (TOP2) bd2[i] = __builtin_amdgcn_fmed3f(bd[i], d, bd2[i]);
-        bd[i] = fminf(bd[i], d);
-        bk[i] = better ? j : bk[i];
-      }
+      eval(t, bl);
     }
+  }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) lab[i] = v[i] ? bk[i] : -1;

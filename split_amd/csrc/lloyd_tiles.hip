@@ -248,7 +248,10 @@ struct BoundedWalker {
   int nbm = 0;
   const float* corr = nullptr;
 
-  // Re-assigns up to 256 pixels (idx/v, previous labels old).
+  // Re-assigns up to 256 pixels (idx/v, previous labels old); PRUNE: the
+  // box-pruned assignment (whole tiles), else every centroid (queued
+  // batches, whose pixels come from all over the image).
+  template <bool PRUNE = true>
   __device__ __forceinline__ void assign(const int64_t (&idx)[4],
                                          const bool (&v)[4], bool lvec,
                                          const int (&old)[4]) {
@@ -262,8 +265,8 @@ struct BoundedWalker {
       commit_assignment<false, true>(a, fl, qs, lane, idx, v, lvec, old, x,
                                      y, z, lab, slack, slice, run);
     } else {
-      reassign<CPL, false>(a, cx, cy, cz, cn, fl, qs, lane, idx, v, lvec,
-                           old, slice, run);
+      reassign<CPL, false, PRUNE>(a, cx, cy, cz, cn, fl, qs, lane, idx, v,
+                                  lvec, old, slice, run);
     }
   }
 
@@ -314,7 +317,7 @@ struct BoundedWalker {
       idx[i] = qidx[v[i] ? e : 0];
       old[i] = v[i] ? qold[e] : -1;
     }
-    assign(idx, v, false, old);
+    assign<false>(idx, v, false, old);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int e = 256 + lane + 64 * i;
