@@ -847,7 +847,8 @@ def main():
         bytes_per_launch = 12 * n
         achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
         kname = ("tiles_step<1, 3, false> (finalise the previous step, skip test, "
-                 "box-pruned re-assignment, exact sums)" if w.k <= 64 else
+                 "re-assignment: whole tiles box-pruned, queued pixels against "
+                 "every centroid, exact sums)" if w.k <= 64 else
                  "tiles_step_ball<3> (finalise the previous step, skip test, "
                  "neighbour-ball re-assignment, exact sums)")
         out["roofline"] = {
