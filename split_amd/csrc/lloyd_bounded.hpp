@@ -151,7 +151,7 @@ __device__ __forceinline__ void commit_assignment(
 // Re-assigns up to 256 pixels (4 per lane: idx/v) whose previous labels are
 // old[] (-1: none), writes their new state, and accumulates: every pixel
 // (FULL) or the +/- changes of label (bounded iterations).
-template <int CPL, bool FULL, bool PRUNE = true>
+template <int CPL, bool FULL, bool PRUNE = true, bool COUNTED = false>
 __device__ __forceinline__ void reassign(
     const TileArgs& a, const float (&cx)[CPL], const float (&cy)[CPL],
     const float (&cz)[CPL], const float (&cn)[CPL], const float* fl,
@@ -161,8 +161,8 @@ __device__ __forceinline__ void reassign(
   load_colours(a, idx, v, lvec, x, y, z);
   int lab[4];
   float slack[4];
-  assign_segment<CPL, true, PRUNE>(cx, cy, cz, cn, a.k, lane, x, y, z, v, lab,
-                                   slack);
+  assign_segment<CPL, true, PRUNE, COUNTED>(cx, cy, cz, cn, a.k, lane, x, y,
+                                            z, v, lab, slack);
   commit_assignment<FULL>(a, fl, qs, lane, idx, v, lvec, old, x, y, z,
                           lab, slack, slice, run);
 }

@@ -265,8 +265,8 @@ struct BoundedWalker {
       commit_assignment<false, true>(a, fl, qs, lane, idx, v, lvec, old, x,
                                      y, z, lab, slack, slice, run);
     } else {
-      reassign<CPL, false, PRUNE>(a, cx, cy, cz, cn, fl, qs, lane, idx, v,
-                                  lvec, old, slice, run);
+      reassign<CPL, false, PRUNE, true>(a, cx, cy, cz, cn, fl, qs, lane, idx,
+                                        v, lvec, old, slice, run);
     }
   }
 
