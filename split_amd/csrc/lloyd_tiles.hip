@@ -541,13 +541,15 @@ __device__ __forceinline__ void build_lists(const TileArgs& a, int s, Cent c,
 // Which of the two kernels enqueued for a bounded launch s of a run with
 // neighbour lists runs the step (the other returns after the
 // finalisation): the ball walk when the largest list correction is below
-// 0.6 of the mean nearest-neighbour distance in the lists, else the
+// 0.85 of the mean nearest-neighbour distance in the lists, else the
 // box-pruned step.  A walk visits its label's list until the entries are
 // farther than the correction plus the pixel's distance to its centroid;
 // with corrections near the spacing of the centroids it reaches the end of
 // the list and rescans all k.  On the 8192^2 k = 256 image the ratio is
 // 1.74 / 0.80 / 0.46 / 0.34 in launches 2 / 3 / 4 / 5, where the box step
-// takes 2.9 / 2.8 / 2.6 / 2.6 ms and the walk 8.8 / 4.0 / 1.9 / 1.6 ms.
+// takes 2.9 / 2.8 / 2.6 / 2.6 ms and the walk 8.8 / 4.0 / 1.9 / 1.6 ms; with
+// the branch-free walk steps launch 3 walks too (0.72 ms less per run than
+// at a 0.6 threshold).
 // Every block of both kernels computes it from the same data (corr from the
 // finalisation, the lists of launch s - 1) in the same order of operations
 // (wave 0: per-lane sums in index order, then a fixed DPP reduction; no
@@ -572,7 +574,7 @@ __device__ __forceinline__ bool prefer_ball(const TileArgs& a, int s,
     }
   }
   __syncthreads();
-  const bool ball = red[0] < 0.6f * (red[1] / (float)k);
+  const bool ball = red[0] < 0.85f * (red[1] / (float)k);
   __syncthreads();  // red is free again
   return ball;
 }
