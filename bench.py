@@ -707,8 +707,9 @@ def cpu_baseline(img, width, height, w, gpu_iters, crop):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(see DESIGN.md for how the path shards)"}))
+        print(json.dumps({"error": "the segmentation path runs on one GPU here; "
+                                   "--gpus must be 1 (multi-GPU sharding is not "
+                                   "implemented)"}))
         sys.exit(2)
 
     import numpy as np

@@ -63,8 +63,8 @@ _lib_ref = None
 
 # The reference's host flags (split's common.pri:27-32): the CPU baseline's
 # all-cores timing build.  -march=native must match the CPU that runs it, so
-# this build is made where it is used, into a temporary directory, and never
-# shipped.
+# that library is compiled where it is used, into a temporary directory, and
+# never shipped.
 REFERENCE_FLAGS = ["-Ofast", "-march=native", "-fopenmp", "-mfma", "-mavx2"]
 
 

@@ -8,7 +8,7 @@
  * whatever the thread count.  Arithmetic is IEEE double with FMA contraction
  * disabled at build time (-ffp-contract=off), and the transcendental calls
  * (log, cos, sqrt) are glibc's, so the same image is produced on any host
- * that runs this build.
+ * that runs this generator.
  *
  * Layout of an image: three float planes of W*H pixels (SoA, row-major).
  */

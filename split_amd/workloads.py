@@ -4,7 +4,7 @@ The images come from the host C generator (split_amd/csrc/gen.c, built into
 split_amd/_lib/libsplit_gen.so): Voronoi colour regions, optionally
 multiplied by Lambertian shading of a value-noise height map, plus noise,
 clamped to [0, 1].  Identical parameters give bit-identical images on every
-machine that runs this build.
+machine that runs the generator.
 
 Workloads (name, size, clustering, connected components):
   plain_512   512x512, 64 sites, noise sigma 0.02, k=8, exactly 20 Lloyd
